@@ -1,0 +1,123 @@
+/*
+ * vmbd.h -- C ABI of the MI355X (gfx950) EFF/OLA operator library
+ * (libvmbd_hip.so).
+ *
+ * Every entry point takes raw device pointers (float32, C order), sizes and a
+ * HIP stream (hipStream_t passed as void*), enqueues its work asynchronously
+ * on that stream and returns 0 on success or a negative VMBD_E* code; the
+ * message of the last failure on the calling thread is in vmbd_last_error().
+ * No entry point synchronises the device.
+ *
+ * Reference interfaces replaced (files of the reference repository):
+ *   plan + windows      imagetools.py:398-454 (win2winaux), olaGPU.py:95-166
+ *   operator            olaGPU.py:170-311 (OlaGPU.cnv / OlaGPU.cnvtp) and the
+ *                       kernels they launch in gputools.cu:16-314 plus the
+ *                       pyfft plan (olaGPU.py:155-157, 813-824)
+ */
+#ifndef VMBD_H
+#define VMBD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VMBD_OK 0
+#define VMBD_E_ARG (-1)     /* bad argument / shape                       */
+#define VMBD_E_HIP (-2)     /* HIP runtime error                          */
+#define VMBD_E_NOMEM (-3)   /* device allocation failed                   */
+#define VMBD_E_UNSUP (-4)   /* geometry outside what the kernels support  */
+
+/* Output-size modes of olaGPU.py:127-136 (circ is not supported). */
+#define VMBD_MODE_VALID 0
+#define VMBD_MODE_SAME 1
+#define VMBD_MODE_FULL 2
+
+/* Which operand an OlaGPU instance caches (olaGPU.py:104-118). */
+#define VMBD_CACHE_PSF 0   /* 'F' instance: spectra of the PSF stack          */
+#define VMBD_CACHE_IMAGE 1 /* 'X' instance: spectra of the windowed patches    */
+
+typedef void* vmbd_stream_t; /* hipStream_t */
+
+/* Patch-grid geometry (imagetools.py:423-436, olaGPU.py:127-141).
+ * Image x is H x W; PSF stack f is (csf0*csf1) x sfh x sfw; patch
+ * k = m*csf1 + n covers rows [m*nhop0, m*nhop0+sw0) and columns
+ * [n*nhop1, n*nhop1+sw1); every patch is transformed on a Th x Tw tile with
+ * Th >= sw0+sfh-1, Tw >= sw1+sfw-1, both powers of two in [16, 512]. */
+typedef struct vmbd_geom {
+  int32_t H, W;
+  int32_t sfh, sfw;
+  int32_t csf0, csf1;
+  int32_t sw0, sw1;
+  int32_t nhop0, nhop1;
+  int32_t Th, Tw;
+  int32_t mode;
+} vmbd_geom;
+
+typedef struct vmbd_plan vmbd_plan;
+
+/* Library version (major*10000 + minor*100 + patch). */
+int vmbd_version(void);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* vmbd_last_error(void);
+
+/* Create a plan on the current HIP device.
+ * win_table: host array (n_win, sw0, sw1) of float32 windows ws_k;
+ * win_idx:   host array (csf0*csf1) mapping patch k to its window.
+ * Replaces win2winaux's device windows (imagetools.py:439-454) and the
+ * OlaGPU constructor's set-up (olaGPU.py:95-166).  Returns NULL on error. */
+vmbd_plan* vmbd_plan_create(const vmbd_geom* geom, const float* win_table, int32_t n_win,
+                            const int32_t* win_idx);
+void vmbd_plan_destroy(vmbd_plan* plan);
+
+/* Device bytes held by the plan (workspaces, windows, twiddles). */
+size_t vmbd_plan_workspace_bytes(const vmbd_plan* plan);
+/* Output size of the forward operator for the plan's mode (olaGPU.py:127-136). */
+int vmbd_plan_out_shape(const vmbd_plan* plan, int32_t* sy0, int32_t* sy1);
+/* Number of complex64 elements of one cached spectrum stack,
+ * (csf0*csf1) * (Tw/2+1) * Th, for vmbd_spectra_*. */
+size_t vmbd_plan_spectrum_elems(const vmbd_plan* plan);
+
+/* y = B(x, f) = F x = X f.  x: H x W, f: P x sfh x sfw, y: sy0 x sy1.
+ * Replaces OlaGPU.cnv (olaGPU.py:170-257) with neither operand cached. */
+int vmbd_ola_forward(vmbd_plan* plan, const float* x, const float* f, float* y, vmbd_stream_t stream);
+
+/* xbar = F' g (transpose with respect to the image).  g: sy0 x sy1,
+ * xbar: H x W.  exact=1: true adjoint of vmbd_ola_forward; exact=0: the
+ * reference's patch-local transpose (olaGPU.py:276-311, residual chopped with
+ * patch size sw; OLA extent sw). */
+int vmbd_ola_adjoint_x(vmbd_plan* plan, const float* g, const float* f, float* xbar, int32_t exact,
+                       vmbd_stream_t stream);
+
+/* fbar = X' g (transpose with respect to the PSFs), fbar: P x sfh x sfw.
+ * Replaces OlaGPU.cnvtp of an 'X' instance (olaGPU.py:276-302). */
+int vmbd_ola_adjoint_f(vmbd_plan* plan, const float* g, const float* x, float* fbar, int32_t exact,
+                       vmbd_stream_t stream);
+
+/* Both transposes from one residual (shares the residual spectra). */
+int vmbd_ola_adjoint_xf(vmbd_plan* plan, const float* g, const float* x, const float* f, float* xbar, float* fbar,
+                        int32_t exact, vmbd_stream_t stream);
+
+/* Cached spectra, the OlaGPU constructor (olaGPU.py:142-157):
+ * kind VMBD_CACHE_PSF  : z = f stack  -> spec = FFT(pad_stack(f))
+ * kind VMBD_CACHE_IMAGE: z = image x  -> spec = FFT(chop_mod_pad(x, ws))
+ * spec: complex64 device buffer of vmbd_plan_spectrum_elems() elements. */
+int vmbd_spectra(vmbd_plan* plan, int32_t kind, const float* z, void* spec, vmbd_stream_t stream);
+
+/* OlaGPU.cnv with the cached operand (olaGPU.py:170-257):
+ * kind PSF: u = image x; kind IMAGE: u = PSF stack.  y: sy0 x sy1. */
+int vmbd_ola_forward_cached(vmbd_plan* plan, int32_t kind, const void* spec, const float* u, float* y,
+                            vmbd_stream_t stream);
+
+/* OlaGPU.cnvtp with the cached operand (olaGPU.py:261-311):
+ * kind PSF -> out = xbar (H x W); kind IMAGE -> out = fbar (P x sfh x sfw). */
+int vmbd_ola_adjoint_cached(vmbd_plan* plan, int32_t kind, const void* spec, const float* g, float* out,
+                            int32_t exact, vmbd_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VMBD_H */

@@ -1,0 +1,92 @@
+"""ctypes binding of libvmbd_hip.so (declarations in include/vmbd.h).
+
+torch is imported first so that the library binds to the HIP runtime torch
+already loaded (both carry the soname libamdhip64.so.7).  A missing library
+is an error: there is no fallback path.
+"""
+import ctypes
+import os
+
+import torch  # noqa: F401  (must precede the CDLL load, see module docstring)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, 'libvmbd_hip.so')
+
+VMBD_OK = 0
+MODES = {'valid': 0, 'same': 1, 'full': 2}
+CACHE_PSF = 0
+CACHE_IMAGE = 1
+
+
+class VmbdGeom(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in (
+        'H', 'W', 'sfh', 'sfw', 'csf0', 'csf1', 'sw0', 'sw1',
+        'nhop0', 'nhop1', 'Th', 'Tw', 'mode')]
+
+
+_lib = None
+
+_P = ctypes.c_void_p
+_F = ctypes.c_void_p  # device float*
+_I = ctypes.c_int32
+
+# name -> (restype, argtypes); every name is exported by the library
+SIGNATURES = {
+    'vmbd_version': (ctypes.c_int, []),
+    'vmbd_last_error': (ctypes.c_char_p, []),
+    'vmbd_plan_create': (_P, [ctypes.POINTER(VmbdGeom), ctypes.c_void_p, _I, ctypes.c_void_p]),
+    'vmbd_plan_destroy': (None, [_P]),
+    'vmbd_plan_workspace_bytes': (ctypes.c_size_t, [_P]),
+    'vmbd_plan_out_shape': (ctypes.c_int, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+    'vmbd_plan_spectrum_elems': (ctypes.c_size_t, [_P]),
+    'vmbd_ola_forward': (ctypes.c_int, [_P, _F, _F, _F, _P]),
+    'vmbd_ola_adjoint_x': (ctypes.c_int, [_P, _F, _F, _F, _I, _P]),
+    'vmbd_ola_adjoint_f': (ctypes.c_int, [_P, _F, _F, _F, _I, _P]),
+    'vmbd_ola_adjoint_xf': (ctypes.c_int, [_P, _F, _F, _F, _F, _F, _I, _P]),
+    'vmbd_spectra': (ctypes.c_int, [_P, _I, _F, _P, _P]),
+    'vmbd_ola_forward_cached': (ctypes.c_int, [_P, _I, _P, _F, _F, _P]),
+    'vmbd_ola_adjoint_cached': (ctypes.c_int, [_P, _I, _P, _F, _F, _I, _P]),
+}
+
+
+def lib():
+    """The loaded library (loaded on first use)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f'{LIB_PATH} is missing: build it with `python -m vmbd_amd.build` '
+                '(the HIP extension is required; there is no CPU fallback)')
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+class VmbdError(RuntimeError):
+    pass
+
+
+def check(rc, what=''):
+    if rc != VMBD_OK:
+        msg = lib().vmbd_last_error().decode(errors='replace')
+        raise VmbdError(f'{what}: vmbd error {rc}: {msg}')
+
+
+def last_error():
+    return lib().vmbd_last_error().decode(errors='replace')
+
+
+def ptr(t):
+    """Device pointer of a contiguous tensor (None -> NULL)."""
+    if t is None:
+        return None
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def stream_ptr(stream=None):
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)

@@ -1,0 +1,470 @@
+// C ABI of the EFF/OLA operator: plan object + operator entry points.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/vmbd.h"
+#include "capi_util.h"
+#include "fft_core.h"
+#include "ola_kernels.h"
+
+using namespace vmbd;
+
+struct vmbd_plan {
+  Geo g;
+  int device;
+  float2* tw = nullptr;
+  float* win = nullptr;
+  int* win_idx = nullptr;
+  int n_win = 0;
+  float2* wsG = nullptr;  // [P][NB][ext0]
+  float2* wsX = nullptr;  // [P][NB][sw0]
+  float2* wsF = nullptr;  // [P][NB][sf0]
+  float2* Y1 = nullptr;   // [P][ext0][NB]
+  float2* Y2 = nullptr;   // [P][sf0][NB]
+  size_t bytes = 0;
+  int mc_fwd = 0, mc_adj = 0;
+};
+
+namespace {
+
+bool pow2_in_range(int v) { return v >= 16 && v <= 512 && (v & (v - 1)) == 0; }
+
+// max number of patch rows whose extent covers one coordinate
+int max_cover(int csf, int hop, int ext, int lo, int hi) {
+  int best = 0;
+  for (int u = lo; u < hi; ++u) {
+    if (u < 0) continue;
+    int mhi = std::min(csf - 1, u / hop);
+    int l = u - ext + 1;
+    int mlo = l <= 0 ? 0 : (l + hop - 1) / hop;
+    best = std::max(best, mhi - mlo + 1);
+  }
+  return best;
+}
+
+template <class T>
+bool dev_alloc(T** p, size_t n, size_t& total) {
+  if (n == 0) n = 1;
+  if (hipMalloc((void**)p, n * sizeof(T)) != hipSuccess) return false;
+  total += n * sizeof(T);
+  return true;
+}
+
+void plan_free(vmbd_plan* p) {
+  if (!p) return;
+  (void)hipFree(p->tw);
+  (void)hipFree(p->win);
+  (void)hipFree(p->win_idx);
+  (void)hipFree(p->wsG);
+  (void)hipFree(p->wsX);
+  (void)hipFree(p->wsF);
+  (void)hipFree(p->Y1);
+  (void)hipFree(p->Y2);
+  delete p;
+}
+
+RowSrc src_image(const vmbd_plan* p, const float* x) {
+  const Geo& g = p->g;
+  RowSrc s{};
+  s.src = x;
+  s.pstride = 0;
+  s.ld = g.W;
+  s.srcH = g.H;
+  s.srcW = g.W;
+  s.origin = 1;
+  s.sh0 = s.sh1 = 0;
+  s.rlo = 0;
+  s.rhi = g.sw0;
+  s.clo = 0;
+  s.chi = g.sw1;
+  s.win = p->win;
+  s.win_idx = p->win_idx;
+  return s;
+}
+
+RowSrc src_psf(const vmbd_plan* p, const float* f) {
+  const Geo& g = p->g;
+  RowSrc s{};
+  s.src = f;
+  s.pstride = (long long)g.sf0 * g.sf1;
+  s.ld = g.sf1;
+  s.srcH = g.sf0;
+  s.srcW = g.sf1;
+  s.origin = 0;
+  s.sh0 = s.sh1 = 0;
+  s.rlo = 0;
+  s.rhi = g.sf0;
+  s.clo = 0;
+  s.chi = g.sf1;
+  s.win = nullptr;
+  s.win_idx = nullptr;
+  return s;
+}
+
+// residual patch G_k: tile row b holds g[p_k + b - oc]; exact keeps the full
+// correlation support [0, ext), the reference-local form only [oc, oc+sw).
+RowSrc src_residual(const vmbd_plan* p, const float* gres, int exact) {
+  const Geo& g = p->g;
+  RowSrc s{};
+  s.src = gres;
+  s.pstride = 0;
+  s.ld = g.sy1;
+  s.srcH = g.sy0;
+  s.srcW = g.sy1;
+  s.origin = 1;
+  s.sh0 = g.oc0;
+  s.sh1 = g.oc1;
+  if (exact) {
+    s.rlo = 0;
+    s.rhi = g.ext0;
+    s.clo = 0;
+    s.chi = g.ext1;
+  } else {
+    s.rlo = g.oc0;
+    s.rhi = g.oc0 + g.sw0;
+    s.clo = g.oc1;
+    s.chi = g.oc1 + g.sw1;
+  }
+  s.win = nullptr;
+  s.win_idx = nullptr;
+  return s;
+}
+
+ColIn raw_in(const float2* ws, const RowSrc& s) { return ColIn{ws, s.rlo, s.rhi - s.rlo, 0}; }
+ColIn spec_in(const void* spec, int Th) { return ColIn{(const float2*)spec, 0, Th, 1}; }
+
+float inv_scale(const Geo& g) { return 1.0f / (float)((double)g.Th * (double)g.Tw); }
+
+OlaDst dst_forward(const vmbd_plan* p, float* y) {
+  const Geo& g = p->g;
+  OlaDst d{};
+  d.out = y;
+  d.O0 = g.sy0;
+  d.O1 = g.sy1;
+  d.oc0 = g.oc0;
+  d.oc1 = g.oc1;
+  d.ext0 = g.ext0;
+  d.ext1 = g.ext1;
+  d.src = p->Y1;
+  d.srow = g.ext0;
+  d.win = nullptr;
+  d.win_idx = nullptr;
+  d.mc = p->mc_fwd;
+  return d;
+}
+
+OlaDst dst_adjoint_x(const vmbd_plan* p, float* xbar) {
+  const Geo& g = p->g;
+  OlaDst d{};
+  d.out = xbar;
+  d.O0 = g.H;
+  d.O1 = g.W;
+  d.oc0 = 0;
+  d.oc1 = 0;
+  d.ext0 = g.sw0;
+  d.ext1 = g.sw1;
+  d.src = p->Y1;
+  d.srow = g.sw0;
+  d.win = p->win;
+  d.win_idx = p->win_idx;
+  d.mc = p->mc_adj;
+  return d;
+}
+
+#define VMBD_LAUNCH(expr)                                            \
+  do {                                                               \
+    hipError_t e__ = (expr);                                         \
+    if (e__ != hipSuccess) return vmbd_fail_hip(#expr, e__);         \
+  } while (0)
+
+int check_plan(const vmbd_plan* p) {
+  if (!p) return vmbd_fail(VMBD_E_ARG, "null plan");
+  return VMBD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vmbd_version(void) { return 100; }
+
+vmbd_plan* vmbd_plan_create(const vmbd_geom* gm, const float* win_table, int32_t n_win, const int32_t* win_idx) {
+  vmbd_clear_error();
+  if (!gm || !win_table || !win_idx || n_win <= 0) {
+    vmbd_fail(VMBD_E_ARG, "vmbd_plan_create: null geometry/window arguments");
+    return nullptr;
+  }
+  const vmbd_geom G = *gm;
+  if (G.H <= 0 || G.W <= 0 || G.sfh <= 0 || G.sfw <= 0 || G.csf0 <= 0 || G.csf1 <= 0 || G.sw0 <= 0 ||
+      G.sw1 <= 0 || G.nhop0 <= 0 || G.nhop1 <= 0) {
+    vmbd_fail(VMBD_E_ARG, "vmbd_plan_create: non-positive size in geometry");
+    return nullptr;
+  }
+  if (!pow2_in_range(G.Th) || !pow2_in_range(G.Tw)) {
+    vmbd_fail(VMBD_E_UNSUP, "vmbd_plan_create: tile %dx%d must be powers of two in [16,512]", G.Th, G.Tw);
+    return nullptr;
+  }
+  if (G.sw0 + G.sfh - 1 > G.Th || G.sw1 + G.sfw - 1 > G.Tw) {
+    vmbd_fail(VMBD_E_ARG, "vmbd_plan_create: tile %dx%d smaller than sw+sf-1 = %dx%d", G.Th, G.Tw,
+              G.sw0 + G.sfh - 1, G.sw1 + G.sfw - 1);
+    return nullptr;
+  }
+  Geo g{};
+  g.H = G.H;
+  g.W = G.W;
+  g.sf0 = G.sfh;
+  g.sf1 = G.sfw;
+  g.csf0 = G.csf0;
+  g.csf1 = G.csf1;
+  g.P = G.csf0 * G.csf1;
+  g.sw0 = G.sw0;
+  g.sw1 = G.sw1;
+  g.hop0 = G.nhop0;
+  g.hop1 = G.nhop1;
+  g.Th = G.Th;
+  g.Tw = G.Tw;
+  g.NB = G.Tw / 2 + 1;
+  g.ext0 = G.sw0 + G.sfh - 1;
+  g.ext1 = G.sw1 + G.sfw - 1;
+  switch (G.mode) {
+    case VMBD_MODE_VALID:
+      g.oc0 = G.sfh - 1;
+      g.oc1 = G.sfw - 1;
+      g.sy0 = G.H - G.sfh + 1;
+      g.sy1 = G.W - G.sfw + 1;
+      break;
+    case VMBD_MODE_SAME:
+      g.oc0 = G.sfh / 2;
+      g.oc1 = G.sfw / 2;
+      g.sy0 = G.H;
+      g.sy1 = G.W;
+      break;
+    case VMBD_MODE_FULL:
+      g.oc0 = 0;
+      g.oc1 = 0;
+      g.sy0 = G.H + G.sfh - 1;
+      g.sy1 = G.W + G.sfw - 1;
+      break;
+    default:
+      vmbd_fail(VMBD_E_ARG, "vmbd_plan_create: Not a valid mode!");
+      return nullptr;
+  }
+  if (g.sy0 <= 0 || g.sy1 <= 0) {
+    vmbd_fail(VMBD_E_ARG, "vmbd_plan_create: PSF larger than image");
+    return nullptr;
+  }
+  // every patch origin must lie inside the padded image (sxp >= H, W)
+  const long long sxp0 = (long long)G.nhop0 * (G.csf0 - 1) + G.sw0;
+  const long long sxp1 = (long long)G.nhop1 * (G.csf1 - 1) + G.sw1;
+  if (sxp0 < G.H || sxp1 < G.W) {
+    vmbd_fail(VMBD_E_ARG, "vmbd_plan_create: patch grid covers %lldx%lld < image %dx%d", sxp0, sxp1, G.H, G.W);
+    return nullptr;
+  }
+  for (int k = 0; k < g.P; ++k) {
+    if (win_idx[k] < 0 || win_idx[k] >= n_win) {
+      vmbd_fail(VMBD_E_ARG, "vmbd_plan_create: win_idx[%d]=%d outside [0,%d)", k, win_idx[k], n_win);
+      return nullptr;
+    }
+  }
+  vmbd_plan* p = new vmbd_plan();
+  p->g = g;
+  p->n_win = n_win;
+  (void)hipGetDevice(&p->device);
+  // rows of the OLA gather buffer
+  p->mc_fwd = max_cover(g.csf0, g.hop0, g.ext0, g.oc0, g.oc0 + g.sy0);
+  p->mc_adj = max_cover(g.csf0, g.hop0, g.sw0, 0, g.H);
+  const size_t rb_fwd = (size_t)p->mc_fwd * g.csf1 * g.ext1 * 4;
+  const size_t rb_adj = (size_t)p->mc_adj * g.csf1 * g.sw1 * 4;
+  if (rb_fwd > 96 * 1024 || rb_adj > 96 * 1024) {
+    vmbd_fail(VMBD_E_UNSUP, "vmbd_plan_create: OLA row buffer %zu B exceeds 96 KiB (csf1=%d ext1=%d)",
+              std::max(rb_fwd, rb_adj), g.csf1, g.ext1);
+    delete p;
+    return nullptr;
+  }
+  size_t tot = 0;
+  const size_t P = g.P, NB = g.NB;
+  bool ok = dev_alloc(&p->tw, TW_LEN, tot) && dev_alloc(&p->win, (size_t)n_win * g.sw0 * g.sw1, tot) &&
+            dev_alloc(&p->win_idx, P, tot) && dev_alloc(&p->wsG, P * NB * g.ext0, tot) &&
+            dev_alloc(&p->wsX, P * NB * g.sw0, tot) && dev_alloc(&p->wsF, P * NB * g.sf0, tot) &&
+            dev_alloc(&p->Y1, P * NB * g.ext0, tot) && dev_alloc(&p->Y2, P * NB * g.sf0, tot);
+  if (!ok) {
+    plan_free(p);
+    vmbd_fail(VMBD_E_NOMEM, "vmbd_plan_create: hipMalloc failed");
+    return nullptr;
+  }
+  p->bytes = tot;
+  std::vector<float2> tw(TW_LEN);
+  for (int m = 0; m < TW_LEN; ++m) {
+    const double a = -2.0 * M_PI * (double)m / (double)TW_LEN;
+    tw[m] = make_float2((float)std::cos(a), (float)std::sin(a));
+  }
+  hipError_t e = hipMemcpy(p->tw, tw.data(), TW_LEN * sizeof(float2), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(p->win, win_table, (size_t)n_win * g.sw0 * g.sw1 * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(p->win_idx, win_idx, P * sizeof(int), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    plan_free(p);
+    vmbd_fail_hip("vmbd_plan_create: hipMemcpy", e);
+    return nullptr;
+  }
+  return p;
+}
+
+void vmbd_plan_destroy(vmbd_plan* plan) { plan_free(plan); }
+
+size_t vmbd_plan_workspace_bytes(const vmbd_plan* plan) { return plan ? plan->bytes : 0; }
+
+int vmbd_plan_out_shape(const vmbd_plan* plan, int32_t* sy0, int32_t* sy1) {
+  if (!plan || !sy0 || !sy1) return vmbd_fail(VMBD_E_ARG, "vmbd_plan_out_shape: null argument");
+  *sy0 = plan->g.sy0;
+  *sy1 = plan->g.sy1;
+  return VMBD_OK;
+}
+
+size_t vmbd_plan_spectrum_elems(const vmbd_plan* plan) {
+  if (!plan) return 0;
+  return (size_t)plan->g.P * plan->g.NB * plan->g.Th;
+}
+
+int vmbd_ola_forward(vmbd_plan* p, const float* x, const float* f, float* y, vmbd_stream_t stream) {
+  vmbd_clear_error();
+  if (int r = check_plan(p)) return r;
+  if (!x || !f || !y) return vmbd_fail(VMBD_E_ARG, "vmbd_ola_forward: null pointer");
+  hipStream_t st = (hipStream_t)stream;
+  const Geo& g = p->g;
+  RowSrc sx = src_image(p, x), sf = src_psf(p, f);
+  VMBD_LAUNCH(launch_rows_r2c(g, sx, p->wsX, p->tw, st));
+  VMBD_LAUNCH(launch_rows_r2c(g, sf, p->wsF, p->tw, st));
+  ColOut o1{p->Y1, g.ext0}, o2{nullptr, 0};
+  ColIn none{nullptr, 0, 0, 0};
+  VMBD_LAUNCH(launch_cols(g, COL_FWD, raw_in(p->wsX, sx), raw_in(p->wsF, sf), none, o1, o2, inv_scale(g), nullptr,
+                          p->tw, st));
+  VMBD_LAUNCH(launch_rows_c2r_ola(g, dst_forward(p, y), p->tw, st));
+  return VMBD_OK;
+}
+
+int vmbd_ola_adjoint_x(vmbd_plan* p, const float* gres, const float* f, float* xbar, int32_t exact,
+                       vmbd_stream_t stream) {
+  vmbd_clear_error();
+  if (int r = check_plan(p)) return r;
+  if (!gres || !f || !xbar) return vmbd_fail(VMBD_E_ARG, "vmbd_ola_adjoint_x: null pointer");
+  hipStream_t st = (hipStream_t)stream;
+  const Geo& g = p->g;
+  RowSrc sg = src_residual(p, gres, exact), sf = src_psf(p, f);
+  VMBD_LAUNCH(launch_rows_r2c(g, sg, p->wsG, p->tw, st));
+  VMBD_LAUNCH(launch_rows_r2c(g, sf, p->wsF, p->tw, st));
+  ColOut o1{p->Y1, g.sw0}, o2{nullptr, 0};
+  ColIn none{nullptr, 0, 0, 0};
+  VMBD_LAUNCH(launch_cols(g, COL_ADJ1, raw_in(p->wsG, sg), raw_in(p->wsF, sf), none, o1, o2, inv_scale(g), nullptr,
+                          p->tw, st));
+  VMBD_LAUNCH(launch_rows_c2r_ola(g, dst_adjoint_x(p, xbar), p->tw, st));
+  return VMBD_OK;
+}
+
+int vmbd_ola_adjoint_f(vmbd_plan* p, const float* gres, const float* x, float* fbar, int32_t exact,
+                       vmbd_stream_t stream) {
+  vmbd_clear_error();
+  if (int r = check_plan(p)) return r;
+  if (!gres || !x || !fbar) return vmbd_fail(VMBD_E_ARG, "vmbd_ola_adjoint_f: null pointer");
+  hipStream_t st = (hipStream_t)stream;
+  const Geo& g = p->g;
+  RowSrc sg = src_residual(p, gres, exact), sx = src_image(p, x);
+  VMBD_LAUNCH(launch_rows_r2c(g, sg, p->wsG, p->tw, st));
+  VMBD_LAUNCH(launch_rows_r2c(g, sx, p->wsX, p->tw, st));
+  ColOut o1{p->Y2, g.sf0}, o2{nullptr, 0};
+  ColIn none{nullptr, 0, 0, 0};
+  VMBD_LAUNCH(launch_cols(g, COL_ADJ1, raw_in(p->wsG, sg), raw_in(p->wsX, sx), none, o1, o2, inv_scale(g), nullptr,
+                          p->tw, st));
+  VMBD_LAUNCH(launch_rows_c2r_stack(g, p->Y2, g.sf0, fbar, g.sf0, g.sf1, p->tw, st));
+  return VMBD_OK;
+}
+
+int vmbd_ola_adjoint_xf(vmbd_plan* p, const float* gres, const float* x, const float* f, float* xbar, float* fbar,
+                        int32_t exact, vmbd_stream_t stream) {
+  vmbd_clear_error();
+  if (int r = check_plan(p)) return r;
+  if (!gres || !x || !f || !xbar || !fbar) return vmbd_fail(VMBD_E_ARG, "vmbd_ola_adjoint_xf: null pointer");
+  hipStream_t st = (hipStream_t)stream;
+  const Geo& g = p->g;
+  RowSrc sg = src_residual(p, gres, exact), sx = src_image(p, x), sf = src_psf(p, f);
+  VMBD_LAUNCH(launch_rows_r2c(g, sg, p->wsG, p->tw, st));
+  VMBD_LAUNCH(launch_rows_r2c(g, sx, p->wsX, p->tw, st));
+  VMBD_LAUNCH(launch_rows_r2c(g, sf, p->wsF, p->tw, st));
+  ColOut o1{p->Y1, g.sw0}, o2{p->Y2, g.sf0};
+  VMBD_LAUNCH(launch_cols(g, COL_ADJ2, raw_in(p->wsG, sg), raw_in(p->wsF, sf), raw_in(p->wsX, sx), o1, o2,
+                          inv_scale(g), nullptr, p->tw, st));
+  VMBD_LAUNCH(launch_rows_c2r_ola(g, dst_adjoint_x(p, xbar), p->tw, st));
+  VMBD_LAUNCH(launch_rows_c2r_stack(g, p->Y2, g.sf0, fbar, g.sf0, g.sf1, p->tw, st));
+  return VMBD_OK;
+}
+
+int vmbd_spectra(vmbd_plan* p, int32_t kind, const float* z, void* spec, vmbd_stream_t stream) {
+  vmbd_clear_error();
+  if (int r = check_plan(p)) return r;
+  if (!z || !spec) return vmbd_fail(VMBD_E_ARG, "vmbd_spectra: null pointer");
+  if (kind != VMBD_CACHE_PSF && kind != VMBD_CACHE_IMAGE) return vmbd_fail(VMBD_E_ARG, "vmbd_spectra: bad kind");
+  hipStream_t st = (hipStream_t)stream;
+  const Geo& g = p->g;
+  RowSrc s = kind == VMBD_CACHE_PSF ? src_psf(p, z) : src_image(p, z);
+  float2* ws = kind == VMBD_CACHE_PSF ? p->wsF : p->wsX;
+  VMBD_LAUNCH(launch_rows_r2c(g, s, ws, p->tw, st));
+  ColOut none_o{nullptr, 0};
+  ColIn none{nullptr, 0, 0, 0};
+  VMBD_LAUNCH(launch_cols(g, COL_SPEC, raw_in(ws, s), none, none, none_o, none_o, 1.f, (float2*)spec, p->tw, st));
+  return VMBD_OK;
+}
+
+int vmbd_ola_forward_cached(vmbd_plan* p, int32_t kind, const void* spec, const float* u, float* y,
+                            vmbd_stream_t stream) {
+  vmbd_clear_error();
+  if (int r = check_plan(p)) return r;
+  if (!spec || !u || !y) return vmbd_fail(VMBD_E_ARG, "vmbd_ola_forward_cached: null pointer");
+  if (kind != VMBD_CACHE_PSF && kind != VMBD_CACHE_IMAGE)
+    return vmbd_fail(VMBD_E_ARG, "vmbd_ola_forward_cached: bad kind");
+  hipStream_t st = (hipStream_t)stream;
+  const Geo& g = p->g;
+  // the free operand is the image for a PSF cache and the PSFs for an image cache
+  RowSrc s = kind == VMBD_CACHE_PSF ? src_image(p, u) : src_psf(p, u);
+  float2* ws = kind == VMBD_CACHE_PSF ? p->wsX : p->wsF;
+  VMBD_LAUNCH(launch_rows_r2c(g, s, ws, p->tw, st));
+  ColOut o1{p->Y1, g.ext0}, o2{nullptr, 0};
+  ColIn none{nullptr, 0, 0, 0};
+  VMBD_LAUNCH(launch_cols(g, COL_FWD, raw_in(ws, s), spec_in(spec, g.Th), none, o1, o2, inv_scale(g), nullptr, p->tw,
+                          st));
+  VMBD_LAUNCH(launch_rows_c2r_ola(g, dst_forward(p, y), p->tw, st));
+  return VMBD_OK;
+}
+
+int vmbd_ola_adjoint_cached(vmbd_plan* p, int32_t kind, const void* spec, const float* gres, float* out,
+                            int32_t exact, vmbd_stream_t stream) {
+  vmbd_clear_error();
+  if (int r = check_plan(p)) return r;
+  if (!spec || !gres || !out) return vmbd_fail(VMBD_E_ARG, "vmbd_ola_adjoint_cached: null pointer");
+  if (kind != VMBD_CACHE_PSF && kind != VMBD_CACHE_IMAGE)
+    return vmbd_fail(VMBD_E_ARG, "vmbd_ola_adjoint_cached: bad kind");
+  hipStream_t st = (hipStream_t)stream;
+  const Geo& g = p->g;
+  RowSrc sg = src_residual(p, gres, exact);
+  VMBD_LAUNCH(launch_rows_r2c(g, sg, p->wsG, p->tw, st));
+  ColIn none{nullptr, 0, 0, 0};
+  ColOut o2{nullptr, 0};
+  if (kind == VMBD_CACHE_PSF) {
+    ColOut o1{p->Y1, g.sw0};
+    VMBD_LAUNCH(launch_cols(g, COL_ADJ1, raw_in(p->wsG, sg), spec_in(spec, g.Th), none, o1, o2, inv_scale(g),
+                            nullptr, p->tw, st));
+    VMBD_LAUNCH(launch_rows_c2r_ola(g, dst_adjoint_x(p, out), p->tw, st));
+  } else {
+    ColOut o1{p->Y2, g.sf0};
+    VMBD_LAUNCH(launch_cols(g, COL_ADJ1, raw_in(p->wsG, sg), spec_in(spec, g.Th), none, o1, o2, inv_scale(g),
+                            nullptr, p->tw, st));
+    VMBD_LAUNCH(launch_rows_c2r_stack(g, p->Y2, g.sf0, out, g.sf0, g.sf1, p->tw, st));
+  }
+  return VMBD_OK;
+}
+
+}  // extern "C"

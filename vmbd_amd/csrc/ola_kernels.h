@@ -1,0 +1,83 @@
+// Shared declarations of the EFF/OLA kernels (geometry + launch helpers).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace vmbd {
+
+// Patch-grid geometry of one operator instance, passed by value to kernels.
+//   image x: H x W;  PSF stack f: P x sf0 x sf1;  patch k = m*csf1 + n at
+//   origin (m*hop0, n*hop1);  FFT tile Th x Tw;  half spectrum NB = Tw/2+1.
+//   ext = sw + sf - 1 (support of one patch's linear convolution);
+//   output y: sy0 x sy1 = crop of the OLA buffer at offset (oc0, oc1).
+struct Geo {
+  int H, W;
+  int sf0, sf1;
+  int csf0, csf1, P;
+  int sw0, sw1;
+  int hop0, hop1;
+  int Th, Tw, NB;
+  int ext0, ext1;
+  int oc0, oc1;
+  int sy0, sy1;
+};
+
+// Source of the rows fed to the row transform (chop / pad / window fused).
+// Tile element (r, c) of patch p reads src[p*pstride + (o0+r)*ld + (o1+c)]
+// with (o0, o1) = (origin ? (m*hop0, n*hop1) : (0,0)) - (sh0, sh1), zero when
+// (r, c) is outside [rlo,rhi) x [clo,chi) or the source index is outside
+// [0,srcH) x [0,srcW); multiplied by win[win_idx[p]][r][c] when win != null.
+struct RowSrc {
+  const float* src;
+  long long pstride;
+  int ld, srcH, srcW;
+  int origin;
+  int sh0, sh1;
+  int rlo, rhi, clo, chi;
+  const float* win;
+  const int* win_idx;
+};
+
+// Column-pass input: either raw row-transform output laid out [P][NB][nr]
+// holding tile rows [rlo, rlo+nr) (spec == 0), or a cached spectrum laid out
+// [P][NB][Th] (spec == 1).
+struct ColIn {
+  const float2* p;
+  int rlo, nr, spec;
+};
+
+// Column-pass output rows [0, nrows) in layout [P][nrows][NB].
+struct ColOut {
+  float2* p;
+  int nrows;
+};
+
+// Inverse row transform + overlap-add gather + crop into an output image.
+// Output pixel (i, j) sums, over every patch k whose local extent covers the
+// OLA coordinate (i+oc0, j+oc1), the spatial value at local (r, c) =
+// (i+oc0-m*hop0, j+oc1-n*hop1) taken from src rows [P][srow][NB]; the value is
+// multiplied by win[win_idx[k]][r][c] when win != null.
+struct OlaDst {
+  float* out;
+  int O0, O1;
+  int oc0, oc1;
+  int ext0, ext1;
+  const float2* src;
+  int srow;
+  const float* win;
+  const int* win_idx;
+  int mc;  // max number of patch rows covering one output row
+};
+
+enum ColOp { COL_SPEC = 0, COL_FWD = 1, COL_ADJ1 = 2, COL_ADJ2 = 3 };
+
+// Launchers (ola_kernels.hip).  Return hipError_t of the launch.
+hipError_t launch_rows_r2c(const Geo& g, const RowSrc& s, float2* out, const float2* tw, hipStream_t st);
+hipError_t launch_cols(const Geo& g, int op, const ColIn& a, const ColIn& b, const ColIn& c, const ColOut& o1,
+                       const ColOut& o2, float scale, float2* spec_out, const float2* tw, hipStream_t st);
+hipError_t launch_rows_c2r_ola(const Geo& g, const OlaDst& d, const float2* tw, hipStream_t st);
+hipError_t launch_rows_c2r_stack(const Geo& g, const float2* src, int srow, float* out, int nrows, int ncols,
+                                 const float2* tw, hipStream_t st);
+size_t ola_rowbuf_bytes(const Geo& g, const OlaDst& d);
+
+}  // namespace vmbd
