@@ -117,6 +117,15 @@ int vmbd_ola_forward_cached(vmbd_plan* plan, int32_t kind, const void* spec, con
 int vmbd_ola_adjoint_cached(vmbd_plan* plan, int32_t kind, const void* spec, const float* g, float* out,
                             int32_t exact, vmbd_stream_t stream);
 
+/* Opt-in per-kernel timing (no counterpart in the reference, whose only
+ * timing is wall-clock prints, stopwatch.py:5-22).  When enabled, every kernel
+ * launch is bracketed by HIP events on its stream; vmbd_prof_report()
+ * synchronises them and writes "name count total_us avg_us" lines into buf
+ * and returns the bytes needed.  vmbd_prof_reset() drops the records. */
+int vmbd_prof_enable(int32_t on);
+int vmbd_prof_reset(void);
+int vmbd_prof_report(char* buf, int32_t len);
+
 #ifdef __cplusplus
 }
 #endif

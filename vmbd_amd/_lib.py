@@ -46,6 +46,9 @@ SIGNATURES = {
     'vmbd_spectra': (ctypes.c_int, [_P, _I, _F, _P, _P]),
     'vmbd_ola_forward_cached': (ctypes.c_int, [_P, _I, _P, _F, _F, _P]),
     'vmbd_ola_adjoint_cached': (ctypes.c_int, [_P, _I, _P, _F, _F, _I, _P]),
+    'vmbd_prof_enable': (ctypes.c_int, [_I]),
+    'vmbd_prof_reset': (ctypes.c_int, []),
+    'vmbd_prof_report': (ctypes.c_int, [ctypes.c_char_p, _I]),
 }
 
 
@@ -90,3 +93,24 @@ def ptr(t):
 def stream_ptr(stream=None):
     s = stream if stream is not None else torch.cuda.current_stream()
     return ctypes.c_void_p(s.cuda_stream)
+
+
+def prof_enable(on=True):
+    lib().vmbd_prof_enable(1 if on else 0)
+
+
+def prof_reset():
+    lib().vmbd_prof_reset()
+
+
+def prof_report():
+    """{kernel name: (count, total_us, avg_us)} of the recorded launches."""
+    L = lib()
+    n = L.vmbd_prof_report(None, 0)
+    buf = ctypes.create_string_buffer(max(1, n))
+    L.vmbd_prof_report(buf, n)
+    out = {}
+    for ln in buf.value.decode().splitlines():
+        name, cnt, tot, avg = ln.split()
+        out[name] = (int(cnt), float(tot), float(avg))
+    return out

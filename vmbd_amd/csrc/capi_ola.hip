@@ -155,6 +155,7 @@ OlaDst dst_forward(const vmbd_plan* p, float* y) {
   d.win = nullptr;
   d.win_idx = nullptr;
   d.mc = p->mc_fwd;
+  d.alias = ola_single_round(g, d);
   return d;
 }
 
@@ -173,6 +174,7 @@ OlaDst dst_adjoint_x(const vmbd_plan* p, float* xbar) {
   d.win = p->win;
   d.win_idx = p->win_idx;
   d.mc = p->mc_adj;
+  d.alias = ola_single_round(g, d);
   return d;
 }
 
@@ -278,13 +280,15 @@ vmbd_plan* vmbd_plan_create(const vmbd_geom* gm, const float* win_table, int32_t
   // rows of the OLA gather buffer
   p->mc_fwd = max_cover(g.csf0, g.hop0, g.ext0, g.oc0, g.oc0 + g.sy0);
   p->mc_adj = max_cover(g.csf0, g.hop0, g.sw0, 0, g.H);
-  const size_t rb_fwd = (size_t)p->mc_fwd * g.csf1 * g.ext1 * 4;
-  const size_t rb_adj = (size_t)p->mc_adj * g.csf1 * g.sw1 * 4;
-  if (rb_fwd > 96 * 1024 || rb_adj > 96 * 1024) {
-    vmbd_fail(VMBD_E_UNSUP, "vmbd_plan_create: OLA row buffer %zu B exceeds 96 KiB (csf1=%d ext1=%d)",
-              std::max(rb_fwd, rb_adj), g.csf1, g.ext1);
-    delete p;
-    return nullptr;
+  {
+    const size_t s_fwd = ola_shared_bytes(g, dst_forward(p, nullptr));
+    const size_t s_adj = ola_shared_bytes(g, dst_adjoint_x(p, nullptr));
+    if (s_fwd > 160 * 1024 || s_adj > 160 * 1024) {
+      vmbd_fail(VMBD_E_UNSUP, "vmbd_plan_create: OLA gather needs %zu B of LDS > 160 KiB (csf1=%d ext1=%d)",
+                std::max(s_fwd, s_adj), g.csf1, g.ext1);
+      delete p;
+      return nullptr;
+    }
   }
   size_t tot = 0;
   const size_t P = g.P, NB = g.NB;

@@ -137,7 +137,7 @@ struct FftLds {
   static constexpr int floats2 = N + N / 16;  // float2 slots per transform
 };
 
-// w^r for r = 0..R-1 where w = twiddle base (already conjugated for S=+1).
+// w^r for r = 0..R-1 where w = twiddle base.
 template <int R>
 __device__ __forceinline__ void twiddle_powers(float2 w1, float2 w4, float2* p) {
   p[0] = make_float2(1.f, 0.f);
@@ -166,11 +166,49 @@ __device__ __forceinline__ void twiddle_powers(float2 w1, float2 w4, float2* p) 
   }
 }
 
+// Radix plan of a length-N transform after the first radix-16 pass.
+template <int N>
+struct Plan {
+  static constexpr int R1 = (N >= 256) ? 16 : N / 16;  // second pass radix
+  static constexpr int R2 = (N > 256) ? N / 256 : 1;   // third pass radix
+};
+
+// Per-thread forward twiddles of the second pass (span 16): the thread's
+// 16/R1 butterflies j = t + b*TPT use w^(r*(j mod 16)), w = e^{-2 pi i/(16 R1)}.
+// They depend only on t, so a kernel computes them once and reuses them for
+// every transform it performs.  The (optional) third pass loads its twiddles.
+template <int N>
+struct Twiddles {
+  static constexpr int R1 = Plan<N>::R1;
+  static constexpr int NB1 = (R1 > 1) ? 16 / R1 : 1;
+  static constexpr int CNT = (R1 > 1) ? NB1 * (R1 - 1) : 1;
+  float2 w[CNT];
+};
+
+template <int N>
+__device__ __forceinline__ void twiddles_init(Twiddles<N>& T, int t, const float2* __restrict__ tw) {
+  constexpr int TPT = N / 16;
+  constexpr int R1 = Plan<N>::R1;
+  if constexpr (R1 > 1) {
+    constexpr int NB1 = 16 / R1;
+    constexpr int STR = TW_LEN / (16 * R1);
+#pragma unroll
+    for (int b = 0; b < NB1; ++b) {
+      const int k = (t + b * TPT) & 15;
+      float2 p[R1];
+      twiddle_powers<R1>(tw[k * STR], tw[(4 * k * STR) & (TW_LEN - 1)], p);
+#pragma unroll
+      for (int r = 1; r < R1; ++r) T.w[b * (R1 - 1) + r - 1] = p[r];
+    }
+  }
+}
+
 // One Stockham pass after the first (Ns > 1).  Reads 16 values of the slice,
 // applies 16/R butterflies, writes 16 values (or keeps them in registers when
-// LAST: slot m = b + r*(16/R) holds X[t + TPT*m]).
-template <int N, int R, int NS, int S, bool LAST>
-__device__ __forceinline__ void stockham_pass(float2 (&v)[16], float2* sl, int t,
+// LAST: slot m = b + r*(16/R) holds X[t + TPT*m]).  Twiddles come from `pre`
+// (per-thread forward twiddles, conjugated for S=+1) or from the table.
+template <int N, int R, int NS, int S, bool LAST, bool PRE>
+__device__ __forceinline__ void stockham_pass(float2 (&v)[16], float2* sl, int t, const float2* pre,
                                               const float2* __restrict__ tw) {
   constexpr int TPT = N / 16;
   constexpr int NB = 16 / R;
@@ -186,19 +224,23 @@ __device__ __forceinline__ void stockham_pass(float2 (&v)[16], float2* sl, int t
   for (int b = 0; b < NB; ++b) {
     const int j = t + b * TPT;
     const int k = j & (NS - 1);
-    // w = exp(S 2 pi i k / (NS R)) = tw[k * TW_LEN/(NS R)] (conj for S=+1)
-    constexpr int STR = TW_LEN / (NS * R);
-    float2 w1 = tw[k * STR];
-    float2 w4 = tw[(4 * k * STR) & (TW_LEN - 1)];
-    if (S > 0) {
-      w1 = cconj(w1);
-      w4 = cconj(w4);
-    }
-    float2 p[R];
-    twiddle_powers<R>(w1, w4, p);
     float2* q = in + b * R;
+    if constexpr (PRE) {
 #pragma unroll
-    for (int r = 1; r < R; ++r) q[r] = cmul(q[r], p[r]);
+      for (int r = 1; r < R; ++r) q[r] = (S < 0) ? cmul(q[r], pre[b * (R - 1) + r - 1]) : cmulc(q[r], pre[b * (R - 1) + r - 1]);
+    } else {
+      constexpr int STR = TW_LEN / (NS * R);
+      float2 w1 = tw[k * STR];
+      float2 w4 = tw[(4 * k * STR) & (TW_LEN - 1)];
+      if (S > 0) {
+        w1 = cconj(w1);
+        w4 = cconj(w4);
+      }
+      float2 p[R];
+      twiddle_powers<R>(w1, w4, p);
+#pragma unroll
+      for (int r = 1; r < R; ++r) q[r] = cmul(q[r], p[r]);
+    }
     dftR<R, S>(q);
     if constexpr (LAST) {
 #pragma unroll
@@ -216,7 +258,8 @@ __device__ __forceinline__ void stockham_pass(float2 (&v)[16], float2* sl, int t
 // v: input mapping on entry, output mapping on exit.  S=-1 forward, +1 inverse
 // (unscaled).  Must be called by all TPT threads of the transform.
 template <int N, int S>
-__device__ __forceinline__ void fft(float2 (&v)[16], float2* sl, int t, const float2* __restrict__ tw) {
+__device__ __forceinline__ void fft(float2 (&v)[16], float2* sl, int t, const Twiddles<N>& T,
+                                    const float2* __restrict__ tw) {
   static_assert(N >= 16 && N <= 1024 && (N & (N - 1)) == 0, "N must be a power of two in [16,1024]");
   // pass 0: radix 16, Ns = 1, j = t, no twiddles; output index t*16 + r
   dft16<S>(v);
@@ -226,20 +269,13 @@ __device__ __forceinline__ void fft(float2 (&v)[16], float2* sl, int t, const fl
 #pragma unroll
     for (int r = 0; r < 16; ++r) sl[lds_pad(t * 16 + r)] = v[r];
     wave_sync();
-    if constexpr (N == 32) {
-      stockham_pass<N, 2, 16, S, true>(v, sl, t, tw);
-    } else if constexpr (N == 64) {
-      stockham_pass<N, 4, 16, S, true>(v, sl, t, tw);
-    } else if constexpr (N == 128) {
-      stockham_pass<N, 8, 16, S, true>(v, sl, t, tw);
-    } else if constexpr (N == 256) {
-      stockham_pass<N, 16, 16, S, true>(v, sl, t, tw);
-    } else if constexpr (N == 512) {
-      stockham_pass<N, 16, 16, S, false>(v, sl, t, tw);
-      stockham_pass<N, 2, 256, S, true>(v, sl, t, tw);
-    } else {  // 1024
-      stockham_pass<N, 16, 16, S, false>(v, sl, t, tw);
-      stockham_pass<N, 4, 256, S, true>(v, sl, t, tw);
+    constexpr int R1 = Plan<N>::R1;
+    constexpr int R2 = Plan<N>::R2;
+    if constexpr (R2 == 1) {
+      stockham_pass<N, R1, 16, S, true, true>(v, sl, t, T.w, tw);
+    } else {
+      stockham_pass<N, R1, 16, S, false, true>(v, sl, t, T.w, tw);
+      stockham_pass<N, R2, 256, S, true, false>(v, sl, t, nullptr, tw);
     }
   }
 }

@@ -66,7 +66,8 @@ struct OlaDst {
   int srow;
   const float* win;
   const int* win_idx;
-  int mc;  // max number of patch rows covering one output row
+  int mc;     // max number of patch rows covering one output row
+  int alias;  // 1: all transforms of a row fit one round; gather buffer aliases the FFT slices
 };
 
 enum ColOp { COL_SPEC = 0, COL_FWD = 1, COL_ADJ1 = 2, COL_ADJ2 = 3 };
@@ -79,5 +80,7 @@ hipError_t launch_rows_c2r_ola(const Geo& g, const OlaDst& d, const float2* tw, 
 hipError_t launch_rows_c2r_stack(const Geo& g, const float2* src, int srow, float* out, int nrows, int ncols,
                                  const float2* tw, hipStream_t st);
 size_t ola_rowbuf_bytes(const Geo& g, const OlaDst& d);
+size_t ola_shared_bytes(const Geo& g, const OlaDst& d);
+int ola_single_round(const Geo& g, const OlaDst& d);
 
 }  // namespace vmbd

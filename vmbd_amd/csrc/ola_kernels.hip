@@ -2,7 +2,7 @@
 //
 // The operator of olaGPU.py:170-311 is evaluated per patch as a 2-D real FFT
 // convolution on a Th x Tw tile, split in three kernels so that every launch
-// has thousands of workgroups and all global traffic is coalesced:
+// has hundreds to thousands of workgroups and all global traffic is coalesced:
 //
 //   rows_r2c      chop + window + zero pad fused into the load, real row FFTs
 //                 (two rows packed in one complex transform), half spectrum
@@ -10,19 +10,29 @@
 //                 contiguous runs.  Rows that are all zero are never computed.
 //   cols          per (patch, strip of bins): column FFTs of 1..3 inputs, the
 //                 spectrum product (conj for the transposes), inverse column
-//                 FFT, only the output rows that are needed are stored,
+//                 FFT; only the output rows that are needed are stored,
 //                 row-major [P][rows][NB].
 //   rows_c2r_ola  one workgroup per output row: inverse real row FFTs of every
 //                 patch row that lands on it (two patches per transform), then
 //                 a fixed-order overlap-add gather and crop (deterministic,
 //                 no atomics), optional window multiply (x-adjoint).
 //   rows_c2r_stack inverse real row FFTs written as a patch stack (f-adjoint).
+//
+// LDS: each workgroup owns one buffer; the FFT slices and the staging area of
+// the transposed store alias it (separated by workgroup barriers), which keeps
+// the footprint near 35 KB and allows four workgroups per CU.
 #include "fft_core.h"
 #include "ola_kernels.h"
+#include "prof.h"
 
 namespace vmbd {
 
 constexpr int WG = 256;
+
+template <int A, int B>
+struct cmax {
+  static constexpr int v = A > B ? A : B;
+};
 
 // ----------------------------------------------------------------------------
 // rows_r2c
@@ -34,79 +44,87 @@ __global__ __launch_bounds__(WG) void k_rows_r2c(Geo g, RowSrc s, float2* __rest
   constexpr int TPW = WG / TPT;
   constexpr int NBN = TW / 2 + 1;
   constexpr int SL = FftLds<TW>::floats2;
-  __shared__ float2 fl[TPW * SL];
-  __shared__ float2 st[2 * TPW * NBN];
+  __shared__ float2 sm[cmax<TPW * SL, 2 * TPW * NBN>::v];
 
   const int tid = threadIdx.x;
   const int tr = tid / TPT;
   const int t = tid - tr * TPT;
   const int nr = s.rhi - s.rlo;
   const int NQ = (nr + 1) >> 1;
-  const long long nitems = (long long)g.P * NQ;
-  const long long item = (long long)blockIdx.x * TPW + tr;
+  const int nitems = g.P * NQ;
+  const int item = blockIdx.x * TPW + tr;
   const bool active = item < nitems;
-  const int p = active ? (int)(item / NQ) : 0;
-  const int q = active ? (int)(item - (long long)p * NQ) : 0;
+  const int p = active ? item / NQ : 0;
+  const int q = item - p * NQ;
   const int r0 = s.rlo + 2 * q;
   const int m = p / g.csf1;
   const int n = p - m * g.csf1;
   const int o0 = (s.origin ? m * g.hop0 : 0) - s.sh0;
   const int o1 = (s.origin ? n * g.hop1 : 0) - s.sh1;
-  const float* base = s.src + (long long)p * s.pstride;
-  const float* wb = s.win ? s.win + (long long)s.win_idx[p] * g.sw0 * g.sw1 : nullptr;
+  const float* base = s.src + (size_t)p * s.pstride;
+  const float* wb = s.win ? s.win + (size_t)s.win_idx[p] * g.sw0 * g.sw1 : nullptr;
 
+  Twiddles<TW> T;
+  twiddles_init<TW>(T, t, tw);
+
+  // rows r0 (real part) and r0+1 (imaginary part)
+  const int i0 = o0 + r0, i1 = i0 + 1;
+  const bool ok0 = active && r0 < s.rhi && i0 >= 0 && i0 < s.srcH;
+  const bool ok1 = active && r0 + 1 < s.rhi && i1 >= 0 && i1 < s.srcH;
+  const float* row0 = base + (size_t)(ok0 ? i0 : 0) * s.ld;
+  const float* row1 = base + (size_t)(ok1 ? i1 : 0) * s.ld;
+  const float* w0 = wb ? wb + (size_t)r0 * g.sw1 : nullptr;
   float2 v[16];
 #pragma unroll
   for (int rr = 0; rr < 16; ++rr) {
     const int c = t + TPT * rr;
-    float a = 0.f, b = 0.f;
-    if (active && c >= s.clo && c < s.chi) {
-      const int j = o1 + c;
-      if (j >= 0 && j < s.srcW) {
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          const int r = r0 + h;
-          const int i = o0 + r;
-          if (r < s.rhi && i >= 0 && i < s.srcH) {
-            float val = base[(long long)i * s.ld + j];
-            if (wb) val *= wb[r * g.sw1 + c];
-            if (h == 0) a = val; else b = val;
-          }
-        }
-      }
+    const int j = o1 + c;
+    const bool okc = c >= s.clo && c < s.chi && j >= 0 && j < s.srcW;
+    float a = (ok0 && okc) ? row0[j] : 0.f;
+    float b = (ok1 && okc) ? row1[j] : 0.f;
+    if (wb) {
+      a = (ok0 && okc) ? a * w0[c] : 0.f;
+      b = (ok1 && okc) ? b * w0[g.sw1 + c] : 0.f;
     }
     v[rr] = make_float2(a, b);
   }
 
-  float2* sl = fl + tr * SL;
-  fft<TW, -1>(v, sl, t, tw);
+  float2* sl = sm + tr * SL;
+  fft<TW, -1>(v, sl, t, T, tw);
   // natural-order spectrum of z = a + i b into the slice, then split
 #pragma unroll
   for (int mm = 0; mm < 16; ++mm) sl[lds_pad(t + TPT * mm)] = v[mm];
   wave_sync();
+  float2 za[9], zb[9];
+#pragma unroll
+  for (int mm = 0; mm <= 8; ++mm) {
+    const int c = t + TPT * mm;
+    const float2 zc = sl[lds_pad(c < NBN ? c : 0)];
+    const float2 zm = sl[lds_pad((TW - c) & (TW - 1))];
+    // A = (Z[c] + conj Z[-c]) / 2,  B = (Z[c] - conj Z[-c]) / (2i)
+    za[mm] = make_float2(0.5f * (zc.x + zm.x), 0.5f * (zc.y - zm.y));
+    zb[mm] = make_float2(0.5f * (zc.y + zm.y), 0.5f * (zm.x - zc.x));
+  }
+  __syncthreads();  // every slice has been read: the buffer becomes staging
 #pragma unroll
   for (int mm = 0; mm <= 8; ++mm) {
     const int c = t + TPT * mm;
     if (c < NBN) {
-      const float2 zc = sl[lds_pad(c)];
-      const float2 zm = sl[lds_pad((TW - c) & (TW - 1))];
-      // A = (Z[c] + conj Z[-c]) / 2,  B = (Z[c] - conj Z[-c]) / (2i)
-      st[(2 * tr) * NBN + c] = make_float2(0.5f * (zc.x + zm.x), 0.5f * (zc.y - zm.y));
-      st[(2 * tr + 1) * NBN + c] = make_float2(0.5f * (zc.y + zm.y), 0.5f * (zm.x - zc.x));
+      sm[(2 * tr) * NBN + c] = za[mm];
+      sm[(2 * tr + 1) * NBN + c] = zb[mm];
     }
   }
   __syncthreads();
   // bin-major store: consecutive lanes -> consecutive rows of one bin
-  const long long item0 = (long long)blockIdx.x * TPW;
+  const int item0 = blockIdx.x * TPW;
   for (int e = tid; e < 2 * TPW * NBN; e += WG) {
     const int c = e / (2 * TPW);
     const int rl = e - c * (2 * TPW);
-    const long long it = item0 + (rl >> 1);
+    const int it = item0 + (rl >> 1);
     if (it < nitems) {
-      const int pp = (int)(it / NQ);
-      const int qq = (int)(it - (long long)pp * NQ);
-      const int rr = 2 * qq + (rl & 1);
-      if (rr < nr) out[((long long)pp * NBN + c) * nr + rr] = st[rl * NBN + c];
+      const int pp = it / NQ;
+      const int rr = 2 * (it - pp * NQ) + (rl & 1);
+      if (rr < nr) out[((size_t)pp * NBN + c) * nr + rr] = sm[rl * NBN + c];
     }
   }
 }
@@ -118,6 +136,7 @@ static hipError_t rows_r2c_t(const Geo& g, const RowSrc& s, float2* out, const f
   const long long items = (long long)g.P * ((nr + 1) / 2);
   const long long blocks = (items + TPW - 1) / TPW;
   if (blocks <= 0) return hipSuccess;
+  ProfScope ps("rows_r2c", st);
   hipLaunchKernelGGL(k_rows_r2c<TW>, dim3((unsigned)blocks), dim3(WG), 0, st, g, s, out, tw);
   return hipGetLastError();
 }
@@ -141,11 +160,11 @@ template <int TH>
 __device__ __forceinline__ void col_load(float2 (&v)[16], const ColIn& in, int p, int c, int NB, int t, bool active) {
   constexpr int TPT = TH / 16;
   if (in.spec) {
-    const float2* col = in.p + ((long long)p * NB + c) * TH;
+    const float2* col = in.p + ((size_t)p * NB + c) * TH;
 #pragma unroll
     for (int r = 0; r < 16; ++r) v[r] = active ? col[t + TPT * r] : make_float2(0.f, 0.f);
   } else {
-    const float2* col = in.p + ((long long)p * NB + c) * in.nr;
+    const float2* col = in.p + ((size_t)p * NB + c) * in.nr;
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
       const int rr = t + TPT * r - in.rlo;
@@ -154,20 +173,13 @@ __device__ __forceinline__ void col_load(float2 (&v)[16], const ColIn& in, int p
   }
 }
 
-// spectrum of one input in registers (output mapping: slot m = row t+TPT*m)
-template <int TH>
-__device__ __forceinline__ void col_spectrum(float2 (&v)[16], const ColIn& in, int p, int c, int NB, int t,
-                                             bool active, float2* sl, const float2* tw) {
-  col_load<TH>(v, in, p, c, NB, t, active);
-  if (!in.spec) fft<TH, -1>(v, sl, t, tw);
-}
-
-// stage rows [0,nrows) of the strip in LDS and store row-major [P][nrows][NB]
+// stage rows [0,nrows) of the strip in LDS and store row-major [P][nrows][NB].
+// The staging area aliases the FFT slices: barriers on both sides.
 template <int TH, int CS>
 __device__ __forceinline__ void col_store_rows(const float2 (&v)[16], float2* st, const ColOut& o, int p, int strip,
                                                int NB, int cc, int t) {
   constexpr int TPT = TH / 16;
-  __syncthreads();  // staging buffer free
+  __syncthreads();  // all transforms of the workgroup are done with the slices
 #pragma unroll
   for (int m = 0; m < 16; ++m) {
     const int r = t + TPT * m;
@@ -179,8 +191,9 @@ __device__ __forceinline__ void col_store_rows(const float2 (&v)[16], float2* st
   for (int e = threadIdx.x; e < o.nrows * CS; e += WG) {
     const int r = e / CS;
     const int ci = e - r * CS;
-    if (ci < ncol) o.p[((long long)p * o.nrows + r) * NB + c0 + ci] = st[r * (CS + 1) + ci];
+    if (ci < ncol) o.p[((size_t)p * o.nrows + r) * NB + c0 + ci] = st[r * (CS + 1) + ci];
   }
+  __syncthreads();  // staging drained before the slices are reused
 }
 
 template <int TH, int OP>
@@ -189,8 +202,7 @@ __global__ __launch_bounds__(WG) void k_cols(Geo g, ColIn a, ColIn b, ColIn c3, 
   constexpr int TPT = TH / 16;
   constexpr int CS = WG / TPT;  // columns per workgroup
   constexpr int SL = FftLds<TH>::floats2;
-  __shared__ float2 fl[CS * SL];
-  __shared__ float2 st[(OP == COL_SPEC) ? 1 : TH * (CS + 1)];
+  __shared__ float2 sm[(OP == COL_SPEC) ? CS * SL : cmax<CS * SL, TH * (CS + 1)>::v];
 
   const int NB = g.NB;
   const int strips = (NB + CS - 1) / CS;
@@ -200,30 +212,36 @@ __global__ __launch_bounds__(WG) void k_cols(Geo g, ColIn a, ColIn b, ColIn c3, 
   const int t = threadIdx.x - cc * TPT;
   const int c = strip * CS + cc;
   const bool active = c < NB;
-  float2* sl = fl + cc * SL;
+  float2* sl = sm + cc * SL;
+
+  Twiddles<TH> T;
+  twiddles_init<TH>(T, t, tw);
 
   float2 A[16];
-  col_spectrum<TH>(A, a, p, c, NB, t, active, sl, tw);
+  col_load<TH>(A, a, p, c, NB, t, active);
+  if (!a.spec) fft<TH, -1>(A, sl, t, T, tw);
   if constexpr (OP == COL_SPEC) {
     if (active) {
-      float2* col = spec_out + ((long long)p * NB + c) * TH;
+      float2* col = spec_out + ((size_t)p * NB + c) * TH;
 #pragma unroll
       for (int m = 0; m < 16; ++m) col[t + TPT * m] = A[m];
     }
   } else {
     float2 B[16];
-    col_spectrum<TH>(B, b, p, c, NB, t, active, sl, tw);
+    col_load<TH>(B, b, p, c, NB, t, active);
+    if (!b.spec) fft<TH, -1>(B, sl, t, T, tw);
 #pragma unroll
     for (int m = 0; m < 16; ++m)
       B[m] = cscale((OP == COL_FWD) ? cmul(A[m], B[m]) : cmulc(A[m], B[m]), scale);
-    fft<TH, +1>(B, sl, t, tw);
-    col_store_rows<TH, CS>(B, st, o1, p, strip, NB, cc, t);
+    fft<TH, +1>(B, sl, t, T, tw);
+    col_store_rows<TH, CS>(B, sm, o1, p, strip, NB, cc, t);
     if constexpr (OP == COL_ADJ2) {
-      col_spectrum<TH>(B, c3, p, c, NB, t, active, sl, tw);
+      col_load<TH>(B, c3, p, c, NB, t, active);
+      if (!c3.spec) fft<TH, -1>(B, sl, t, T, tw);
 #pragma unroll
       for (int m = 0; m < 16; ++m) B[m] = cscale(cmulc(A[m], B[m]), scale);
-      fft<TH, +1>(B, sl, t, tw);
-      col_store_rows<TH, CS>(B, st, o2, p, strip, NB, cc, t);
+      fft<TH, +1>(B, sl, t, T, tw);
+      col_store_rows<TH, CS>(B, sm, o2, p, strip, NB, cc, t);
     }
   }
 }
@@ -234,6 +252,8 @@ static hipError_t cols_t(const Geo& g, int op, const ColIn& a, const ColIn& b, c
   constexpr int CS = WG / (TH / 16);
   const int strips = (g.NB + CS - 1) / CS;
   const dim3 grid((unsigned)(g.P * strips));
+  static const char* names[4] = {"cols_spec", "cols_fwd", "cols_adj1", "cols_adj2"};
+  ProfScope ps(names[op & 3], st);
   switch (op) {
     case COL_SPEC:
       hipLaunchKernelGGL((k_cols<TH, COL_SPEC>), grid, dim3(WG), 0, st, g, a, b, c, o1, o2, scale, spec_out, tw);
@@ -277,18 +297,11 @@ __device__ __forceinline__ void c2r_load(float2 (&v)[16], const float2* ra, cons
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
     const int c = t + TPT * r;
-    float2 z = make_float2(0.f, 0.f);
-    if (c <= TW / 2) {
-      const float2 A = ra ? ra[c] : make_float2(0.f, 0.f);
-      const float2 B = rb ? rb[c] : make_float2(0.f, 0.f);
-      z = make_float2(A.x - B.y, A.y + B.x);
-    } else {
-      const int cm = TW - c;
-      const float2 A = ra ? ra[cm] : make_float2(0.f, 0.f);
-      const float2 B = rb ? rb[cm] : make_float2(0.f, 0.f);
-      z = make_float2(A.x + B.y, B.x - A.y);
-    }
-    v[r] = z;
+    const bool lo = c <= TW / 2;
+    const int cc = lo ? c : TW - c;
+    const float2 A = ra ? ra[cc] : make_float2(0.f, 0.f);
+    const float2 B = rb ? rb[cc] : make_float2(0.f, 0.f);
+    v[r] = lo ? make_float2(A.x - B.y, A.y + B.x) : make_float2(A.x + B.y, B.x - A.y);
   }
 }
 
@@ -300,8 +313,10 @@ __global__ __launch_bounds__(WG) void k_rows_c2r_ola(Geo g, OlaDst d, const floa
   constexpr int TPT = TW / 16;
   constexpr int TPW = WG / TPT;
   constexpr int SL = FftLds<TW>::floats2;
-  __shared__ float2 fl[TPW * SL];
-  extern __shared__ float rowbuf[];  // [mc][csf1][ext1]
+  extern __shared__ float2 dyn[];
+  float2* fl = dyn;
+  // single-round launches alias the gather buffer with the FFT slices
+  float* rowbuf = d.alias ? (float*)dyn : (float*)(dyn + TPW * SL);
 
   const int i = blockIdx.x;
   const int u = i + d.oc0;
@@ -319,6 +334,9 @@ __global__ __launch_bounds__(WG) void k_rows_c2r_ola(Geo g, OlaDst d, const floa
   const int t = threadIdx.x - tr * TPT;
   float2* sl = fl + tr * SL;
 
+  Twiddles<TW> T;
+  twiddles_init<TW>(T, t, tw);
+
   for (int base = 0; base < NT; base += TPW) {
     const int tix = base + tr;
     const bool act = tix < NT;
@@ -330,17 +348,18 @@ __global__ __launch_bounds__(WG) void k_rows_c2r_ola(Geo g, OlaDst d, const floa
     const int ka = m * g.csf1 + na;
     const bool hasb = nb < g.csf1;
     const bool rok = act && r >= 0 && r < d.srow;
-    const float2* ra = rok ? d.src + ((long long)ka * d.srow + r) * g.NB : nullptr;
-    const float2* rb = (rok && hasb) ? d.src + ((long long)(ka + 1) * d.srow + r) * g.NB : nullptr;
+    const float2* ra = rok ? d.src + ((size_t)ka * d.srow + r) * g.NB : nullptr;
+    const float2* rb = (rok && hasb) ? d.src + ((size_t)(ka + 1) * d.srow + r) * g.NB : nullptr;
     float2 v[16];
     c2r_load<TW>(v, ra, rb, t);
-    fft<TW, +1>(v, sl, t, tw);
+    fft<TW, +1>(v, sl, t, T, tw);
+    if (d.alias) __syncthreads();  // slices are about to be overwritten by rowbuf
     if (act) {
-      const float* wa = (d.win && r < g.sw0) ? d.win + ((long long)d.win_idx[ka] * g.sw0 + r) * g.sw1 : nullptr;
+      const float* wa = (d.win && r < g.sw0) ? d.win + ((size_t)d.win_idx[ka] * g.sw0 + r) * g.sw1 : nullptr;
       const float* wbp =
-          (d.win && hasb && r < g.sw0) ? d.win + ((long long)d.win_idx[ka + 1] * g.sw0 + r) * g.sw1 : nullptr;
-      float* ba = rowbuf + ((long long)mi * g.csf1 + na) * d.ext1;
-      float* bb = rowbuf + ((long long)mi * g.csf1 + nb) * d.ext1;
+          (d.win && hasb && r < g.sw0) ? d.win + ((size_t)d.win_idx[ka + 1] * g.sw0 + r) * g.sw1 : nullptr;
+      float* ba = rowbuf + (mi * g.csf1 + na) * d.ext1;
+      float* bb = rowbuf + (mi * g.csf1 + nb) * d.ext1;
 #pragma unroll
       for (int mm = 0; mm < 16; ++mm) {
         const int col = t + TPT * mm;
@@ -363,12 +382,16 @@ __global__ __launch_bounds__(WG) void k_rows_c2r_ola(Geo g, OlaDst d, const floa
     float acc = 0.f;
     if (vcol >= 0) {
       const int nhi = min(g.csf1 - 1, vcol / g.hop1);
-      const int lo = vcol - d.ext1 + 1;
-      const int nlo = lo <= 0 ? 0 : (lo + g.hop1 - 1) / g.hop1;
-      for (int mi = 0; mi < Mc; ++mi)
-        for (int nn = nlo; nn <= nhi; ++nn) acc += rowbuf[((long long)mi * g.csf1 + nn) * d.ext1 + (vcol - nn * g.hop1)];
+      for (int mi = 0; mi < Mc; ++mi) {
+        const float* rbm = rowbuf + mi * g.csf1 * d.ext1;
+        // covering n are nhi, nhi-1, ... while the local column stays < ext1
+        float sub = 0.f;
+        for (int nn = nhi, lc = vcol - nhi * g.hop1; nn >= 0 && lc < d.ext1; --nn, lc += g.hop1)
+          sub += rbm[nn * d.ext1 + lc];
+        acc += sub;
+      }
     }
-    d.out[(long long)i * d.O1 + j] = acc;
+    d.out[(size_t)i * d.O1 + j] = acc;
   }
 }
 
@@ -376,9 +399,23 @@ size_t ola_rowbuf_bytes(const Geo& g, const OlaDst& d) {
   return (size_t)d.mc * g.csf1 * d.ext1 * sizeof(float);
 }
 
+size_t ola_shared_bytes(const Geo& g, const OlaDst& d) {
+  const int TPT = g.Tw / 16;
+  const int TPW = WG / TPT;
+  const size_t fl = (size_t)TPW * (g.Tw + g.Tw / 16) * sizeof(float2);
+  const size_t rb = ola_rowbuf_bytes(g, d);
+  return d.alias ? (fl > rb ? fl : rb) : fl + rb;
+}
+
+int ola_single_round(const Geo& g, const OlaDst& d) {
+  const int TPW = WG / (g.Tw / 16);
+  return d.mc * ((g.csf1 + 1) / 2) <= TPW;
+}
+
 template <int TW>
 static hipError_t c2r_ola_t(const Geo& g, const OlaDst& d, const float2* tw, hipStream_t st) {
-  const size_t shm = ola_rowbuf_bytes(g, d);
+  const size_t shm = ola_shared_bytes(g, d);
+  ProfScope ps(d.win ? "rows_c2r_ola_win" : "rows_c2r_ola", st);
   hipLaunchKernelGGL(k_rows_c2r_ola<TW>, dim3((unsigned)d.O0), dim3(WG), shm, st, g, d, tw);
   return hipGetLastError();
 }
@@ -410,24 +447,26 @@ __global__ __launch_bounds__(WG) void k_rows_c2r_stack(Geo g, const float2* __re
   const int tr = threadIdx.x / TPT;
   const int t = threadIdx.x - tr * TPT;
   const int NQ = (nrows + 1) >> 1;
-  const long long item = (long long)blockIdx.x * TPW + tr;
-  const bool act = item < (long long)g.P * NQ;
-  const int p = act ? (int)(item / NQ) : 0;
-  const int q = act ? (int)(item - (long long)p * NQ) : 0;
+  const int item = blockIdx.x * TPW + tr;
+  const bool act = item < g.P * NQ;
+  const int p = act ? item / NQ : 0;
+  const int q = item - p * NQ;
   const int r0 = 2 * q, r1 = 2 * q + 1;
-  const float2* ra = act ? src + ((long long)p * srow + r0) * g.NB : nullptr;
-  const float2* rb = (act && r1 < nrows) ? src + ((long long)p * srow + r1) * g.NB : nullptr;
+  const float2* ra = act ? src + ((size_t)p * srow + r0) * g.NB : nullptr;
+  const float2* rb = (act && r1 < nrows) ? src + ((size_t)p * srow + r1) * g.NB : nullptr;
+  Twiddles<TW> T;
+  twiddles_init<TW>(T, t, tw);
   float2 v[16];
   c2r_load<TW>(v, ra, rb, t);
   float2* sl = fl + tr * SL;
-  fft<TW, +1>(v, sl, t, tw);
+  fft<TW, +1>(v, sl, t, T, tw);
   if (act) {
 #pragma unroll
     for (int mm = 0; mm < 16; ++mm) {
       const int col = t + TPT * mm;
       if (col < ncols) {
-        out[((long long)p * nrows + r0) * ncols + col] = v[mm].x;
-        if (r1 < nrows) out[((long long)p * nrows + r1) * ncols + col] = v[mm].y;
+        out[((size_t)p * nrows + r0) * ncols + col] = v[mm].x;
+        if (r1 < nrows) out[((size_t)p * nrows + r1) * ncols + col] = v[mm].y;
       }
     }
   }
@@ -440,6 +479,7 @@ static hipError_t c2r_stack_t(const Geo& g, const float2* src, int srow, float* 
   const long long items = (long long)g.P * ((nrows + 1) / 2);
   const long long blocks = (items + TPW - 1) / TPW;
   if (blocks <= 0) return hipSuccess;
+  ProfScope ps("rows_c2r_stack", st);
   hipLaunchKernelGGL(k_rows_c2r_stack<TW>, dim3((unsigned)blocks), dim3(WG), 0, st, g, src, srow, out, nrows,
                      ncols, tw);
   return hipGetLastError();
