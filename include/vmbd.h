@@ -35,6 +35,11 @@ extern "C" {
 #define VMBD_MODE_SAME 1
 #define VMBD_MODE_FULL 2
 
+/* Flags of vmbd_ola_adjoint_xf. */
+#define VMBD_REUSE_ROWS 1 /* x and f unchanged since the last vmbd_ola_forward on
+                             this plan with the same nframes: reuse its row
+                             transforms of x and f                            */
+
 /* Which operand an OlaGPU instance caches (olaGPU.py:104-118). */
 #define VMBD_CACHE_PSF 0   /* 'F' instance: spectra of the PSF stack          */
 #define VMBD_CACHE_IMAGE 1 /* 'X' instance: spectra of the windowed patches    */
@@ -45,7 +50,11 @@ typedef void* vmbd_stream_t; /* hipStream_t */
  * Image x is H x W; PSF stack f is (csf0*csf1) x sfh x sfw; patch
  * k = m*csf1 + n covers rows [m*nhop0, m*nhop0+sw0) and columns
  * [n*nhop1, n*nhop1+sw1); every patch is transformed on a Th x Tw tile with
- * Th >= sw0+sfh-1, Tw >= sw1+sfw-1, both powers of two in [16, 512]. */
+ * Th >= sw0+sfh-1, Tw >= sw1+sfw-1, both powers of two in [16, 512].
+ * max_frames (>= 1) bounds the frames of one batched call: the operators take
+ * nframes frames of the same geometry stored back to back (x: nframes*H*W,
+ * f: nframes*P*sfh*sfw, y: nframes*sy0*sy1) and process all their patches in
+ * the same launches. */
 typedef struct vmbd_geom {
   int32_t H, W;
   int32_t sfh, sfw;
@@ -54,6 +63,7 @@ typedef struct vmbd_geom {
   int32_t nhop0, nhop1;
   int32_t Th, Tw;
   int32_t mode;
+  int32_t max_frames;
 } vmbd_geom;
 
 typedef struct vmbd_plan vmbd_plan;
@@ -77,45 +87,49 @@ void vmbd_plan_destroy(vmbd_plan* plan);
 size_t vmbd_plan_workspace_bytes(const vmbd_plan* plan);
 /* Output size of the forward operator for the plan's mode (olaGPU.py:127-136). */
 int vmbd_plan_out_shape(const vmbd_plan* plan, int32_t* sy0, int32_t* sy1);
-/* Number of complex64 elements of one cached spectrum stack,
- * (csf0*csf1) * (Tw/2+1) * Th, for vmbd_spectra_*. */
+/* Number of complex64 elements of the cached spectrum stack of one frame,
+ * (csf0*csf1) * (Tw/2+1) * Th, for vmbd_spectra. */
 size_t vmbd_plan_spectrum_elems(const vmbd_plan* plan);
 
-/* y = B(x, f) = F x = X f.  x: H x W, f: P x sfh x sfw, y: sy0 x sy1.
- * Replaces OlaGPU.cnv (olaGPU.py:170-257) with neither operand cached. */
-int vmbd_ola_forward(vmbd_plan* plan, const float* x, const float* f, float* y, vmbd_stream_t stream);
+/* y = B(x, f) = F x = X f for nframes frames.  x: H x W, f: P x sfh x sfw,
+ * y: sy0 x sy1 per frame.  Replaces OlaGPU.cnv (olaGPU.py:170-257) with
+ * neither operand cached. */
+int vmbd_ola_forward(vmbd_plan* plan, int32_t nframes, const float* x, const float* f, float* y,
+                     vmbd_stream_t stream);
 
 /* xbar = F' g (transpose with respect to the image).  g: sy0 x sy1,
  * xbar: H x W.  exact=1: true adjoint of vmbd_ola_forward; exact=0: the
  * reference's patch-local transpose (olaGPU.py:276-311, residual chopped with
  * patch size sw; OLA extent sw). */
-int vmbd_ola_adjoint_x(vmbd_plan* plan, const float* g, const float* f, float* xbar, int32_t exact,
-                       vmbd_stream_t stream);
+int vmbd_ola_adjoint_x(vmbd_plan* plan, int32_t nframes, const float* g, const float* f, float* xbar,
+                       int32_t exact, vmbd_stream_t stream);
 
 /* fbar = X' g (transpose with respect to the PSFs), fbar: P x sfh x sfw.
  * Replaces OlaGPU.cnvtp of an 'X' instance (olaGPU.py:276-302). */
-int vmbd_ola_adjoint_f(vmbd_plan* plan, const float* g, const float* x, float* fbar, int32_t exact,
-                       vmbd_stream_t stream);
+int vmbd_ola_adjoint_f(vmbd_plan* plan, int32_t nframes, const float* g, const float* x, float* fbar,
+                       int32_t exact, vmbd_stream_t stream);
 
-/* Both transposes from one residual (shares the residual spectra). */
-int vmbd_ola_adjoint_xf(vmbd_plan* plan, const float* g, const float* x, const float* f, float* xbar, float* fbar,
-                        int32_t exact, vmbd_stream_t stream);
+/* Both transposes from one residual (shares the residual spectra).  flags:
+ * VMBD_REUSE_ROWS skips the row transforms of x and f (see above). */
+int vmbd_ola_adjoint_xf(vmbd_plan* plan, int32_t nframes, const float* g, const float* x, const float* f,
+                        float* xbar, float* fbar, int32_t exact, int32_t flags, vmbd_stream_t stream);
 
 /* Cached spectra, the OlaGPU constructor (olaGPU.py:142-157):
  * kind VMBD_CACHE_PSF  : z = f stack  -> spec = FFT(pad_stack(f))
  * kind VMBD_CACHE_IMAGE: z = image x  -> spec = FFT(chop_mod_pad(x, ws))
  * spec: complex64 device buffer of vmbd_plan_spectrum_elems() elements. */
-int vmbd_spectra(vmbd_plan* plan, int32_t kind, const float* z, void* spec, vmbd_stream_t stream);
+int vmbd_spectra(vmbd_plan* plan, int32_t nframes, int32_t kind, const float* z, void* spec,
+                 vmbd_stream_t stream);
 
 /* OlaGPU.cnv with the cached operand (olaGPU.py:170-257):
  * kind PSF: u = image x; kind IMAGE: u = PSF stack.  y: sy0 x sy1. */
-int vmbd_ola_forward_cached(vmbd_plan* plan, int32_t kind, const void* spec, const float* u, float* y,
-                            vmbd_stream_t stream);
+int vmbd_ola_forward_cached(vmbd_plan* plan, int32_t nframes, int32_t kind, const void* spec, const float* u,
+                            float* y, vmbd_stream_t stream);
 
 /* OlaGPU.cnvtp with the cached operand (olaGPU.py:261-311):
  * kind PSF -> out = xbar (H x W); kind IMAGE -> out = fbar (P x sfh x sfw). */
-int vmbd_ola_adjoint_cached(vmbd_plan* plan, int32_t kind, const void* spec, const float* g, float* out,
-                            int32_t exact, vmbd_stream_t stream);
+int vmbd_ola_adjoint_cached(vmbd_plan* plan, int32_t nframes, int32_t kind, const void* spec, const float* g,
+                            float* out, int32_t exact, vmbd_stream_t stream);
 
 /* Opt-in per-kernel timing (no counterpart in the reference, whose only
  * timing is wall-clock prints, stopwatch.py:5-22).  When enabled, every kernel

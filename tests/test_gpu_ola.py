@@ -129,3 +129,95 @@ def test_deterministic(cuda):
     a2 = plan.adjoint_xf(gt, xt, ft)
     assert np.array_equal(a1[0].cpu().numpy(), a2[0].cpu().numpy())
     assert np.array_equal(a1[1].cpu().numpy(), a2[1].cpu().numpy())
+
+
+@pytest.mark.parametrize('geom', [GEOMS[1], GEOMS[3], ((300, 700), (2, 3), (25, 25), 0.5)],
+                         ids=lambda g: f'{g[0]}-{g[1]}-{g[2]}')
+def test_batched_frames_match_single(geom, cuda):
+    """B frames in one launch give bitwise the per-frame results, and each
+    frame matches the oracle."""
+    import torch
+    from vmbd_amd.imagetools import win2winaux
+    from vmbd_amd.ola import OlaPlan
+    sx, csf, sf, ov = geom
+    print(f'case batched sx={sx} csf={csf} sf={sf}', flush=True)
+    B = 3
+    wa = win2winaux(sx, csf, ov)
+    pb = OlaPlan(sx, sf, wa, device=cuda, max_frames=B)
+    p1 = OlaPlan(sx, sf, wa, device=cuda, max_frames=1)
+    owa = O.Win2WinAux(sx, csf, ov)
+    ins = [case_inputs(20 + b, sx, csf, sf) for b in range(B)]
+    X = _t(np.stack([i[0] for i in ins]), cuda)
+    F = _t(np.stack([i[1] for i in ins]), cuda)
+    G = _t(np.stack([i[2] for i in ins]), cuda)
+    Y = pb.forward(X, F)
+    XB, FB = pb.adjoint_xf(G, X, F, exact=True)
+    XB2 = pb.adjoint_x(G, F, exact=False)
+    FB2 = pb.adjoint_f(G, X, exact=False)
+    for b in range(B):
+        assert torch.equal(Y[b], p1.forward(X[b], F[b]))
+        xb1, fb1 = p1.adjoint_xf(G[b], X[b], F[b], exact=True)
+        assert torch.equal(XB[b], xb1) and torch.equal(FB[b], fb1)
+        assert torch.equal(XB2[b], p1.adjoint_x(G[b], F[b], exact=False))
+        assert torch.equal(FB2[b], p1.adjoint_f(G[b], X[b], exact=False))
+        x32, f32, g32 = (np.float32(a).astype(np.float64) for a in ins[b])
+        assert rel_l2(Y[b].cpu().numpy(), O.forward(x32, f32, owa)) <= TOL
+        assert rel_l2(XB[b].cpu().numpy(), O.adjoint_x(g32, f32, owa, sx, exact=True)) <= TOL
+        assert rel_l2(FB[b].cpu().numpy(), O.adjoint_f(g32, x32, owa, sf, exact=True)) <= TOL
+    # cached spectra, batched
+    Fs = pb.spectra('F', F)
+    Xs = pb.spectra('X', X)
+    assert torch.equal(pb.forward_cached('F', Fs, X), Y)
+    assert rel_l2(pb.forward_cached('X', Xs, F).cpu().numpy(), Y.cpu().numpy()) <= 1e-6
+    assert torch.equal(pb.adjoint_cached('F', Fs, G, exact=True), XB)
+
+
+def test_row_reuse(cuda):
+    """adjoint_xf after forward on the same tensors reuses the row transforms
+    (same result bitwise); an in-place change of x disables the reuse."""
+    import torch
+    sx, csf, sf = (256, 256), (4, 4), (15, 15)
+    wa, plan, owa = _mk(sx, csf, 0.5, sf, 'valid', cuda)
+    x, f, g = case_inputs(30, sx, csf, sf)
+    xt, ft, gt = _t(x, cuda), _t(f, cuda), _t(g, cuda)
+    ref_x, ref_f = plan.adjoint_xf(gt, xt, ft, reuse=False)
+    plan.forward(xt, ft)
+    rx, rf = plan.adjoint_xf(gt, xt, ft)           # reuses
+    assert torch.equal(rx, ref_x) and torch.equal(rf, ref_f)
+    plan.forward(xt, ft)
+    xt.mul_(2.0)                                    # bumps the version counter
+    rx, rf = plan.adjoint_xf(gt, xt, ft)            # must recompute
+    x2 = np.float32(x).astype(np.float64) * 2.0
+    assert rel_l2(rf.cpu().numpy(), O.adjoint_f(np.float32(g).astype(np.float64), x2, owa, sf, exact=True)) <= TOL
+
+
+def test_graph_capture_replay(cuda):
+    """The forward + both transposes captured in a HIP graph replay to the
+    eager results, with eager work between replays."""
+    import torch
+    sx, csf, sf = (256, 256), (4, 4), (15, 15)
+    wa, plan, owa = _mk(sx, csf, 0.5, sf, 'valid', cuda)
+    x, f, g = case_inputs(31, sx, csf, sf)
+    xt, ft, gt = _t(x, cuda), _t(f, cuda), _t(g, cuda)
+    y = torch.empty(plan.sy, device=cuda)
+    xb = torch.empty(sx, device=cuda)
+    fb = torch.empty((16,) + sf, device=cuda)
+    plan.forward(xt, ft, out=y)
+    plan.adjoint_xf(gt, xt, ft, out_x=xb, out_f=fb)
+    e = (y.clone(), xb.clone(), fb.clone())
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        plan.forward(xt, ft, out=y)
+        plan.adjoint_xf(gt, xt, ft, out_x=xb, out_f=fb)
+        gr = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(gr, stream=s):
+            plan.forward(xt, ft, out=y)
+            plan.adjoint_xf(gt, xt, ft, out_x=xb, out_f=fb)
+    torch.cuda.current_stream().wait_stream(s)
+    for it in range(3):
+        y.zero_(); xb.zero_(); fb.zero_()
+        gr.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(y, e[0]) and torch.equal(xb, e[1]) and torch.equal(fb, e[2]), it
+        _ = (xt * 2).sum().item()  # unrelated eager work between replays

@@ -18,10 +18,13 @@ CACHE_PSF = 0
 CACHE_IMAGE = 1
 
 
+REUSE_ROWS = 1
+
+
 class VmbdGeom(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int32) for n in (
         'H', 'W', 'sfh', 'sfw', 'csf0', 'csf1', 'sw0', 'sw1',
-        'nhop0', 'nhop1', 'Th', 'Tw', 'mode')]
+        'nhop0', 'nhop1', 'Th', 'Tw', 'mode', 'max_frames')]
 
 
 _lib = None
@@ -39,13 +42,13 @@ SIGNATURES = {
     'vmbd_plan_workspace_bytes': (ctypes.c_size_t, [_P]),
     'vmbd_plan_out_shape': (ctypes.c_int, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
     'vmbd_plan_spectrum_elems': (ctypes.c_size_t, [_P]),
-    'vmbd_ola_forward': (ctypes.c_int, [_P, _F, _F, _F, _P]),
-    'vmbd_ola_adjoint_x': (ctypes.c_int, [_P, _F, _F, _F, _I, _P]),
-    'vmbd_ola_adjoint_f': (ctypes.c_int, [_P, _F, _F, _F, _I, _P]),
-    'vmbd_ola_adjoint_xf': (ctypes.c_int, [_P, _F, _F, _F, _F, _F, _I, _P]),
-    'vmbd_spectra': (ctypes.c_int, [_P, _I, _F, _P, _P]),
-    'vmbd_ola_forward_cached': (ctypes.c_int, [_P, _I, _P, _F, _F, _P]),
-    'vmbd_ola_adjoint_cached': (ctypes.c_int, [_P, _I, _P, _F, _F, _I, _P]),
+    'vmbd_ola_forward': (ctypes.c_int, [_P, _I, _F, _F, _F, _P]),
+    'vmbd_ola_adjoint_x': (ctypes.c_int, [_P, _I, _F, _F, _F, _I, _P]),
+    'vmbd_ola_adjoint_f': (ctypes.c_int, [_P, _I, _F, _F, _F, _I, _P]),
+    'vmbd_ola_adjoint_xf': (ctypes.c_int, [_P, _I, _F, _F, _F, _F, _F, _I, _I, _P]),
+    'vmbd_spectra': (ctypes.c_int, [_P, _I, _I, _F, _P, _P]),
+    'vmbd_ola_forward_cached': (ctypes.c_int, [_P, _I, _I, _P, _F, _F, _P]),
+    'vmbd_ola_adjoint_cached': (ctypes.c_int, [_P, _I, _I, _P, _F, _F, _I, _P]),
     'vmbd_prof_enable': (ctypes.c_int, [_I]),
     'vmbd_prof_reset': (ctypes.c_int, []),
     'vmbd_prof_report': (ctypes.c_int, [ctypes.c_char_p, _I]),

@@ -28,6 +28,8 @@ struct vmbd_plan {
   float2* Y2 = nullptr;   // [P][sf0][NB]
   size_t bytes = 0;
   int mc_fwd = 0, mc_adj = 0;
+  int max_frames = 1;
+  int rows_frames = 0;  // >0: wsX/wsF hold the row transforms of the last forward (that many frames)
 };
 
 namespace {
@@ -73,6 +75,7 @@ RowSrc src_image(const vmbd_plan* p, const float* x) {
   RowSrc s{};
   s.src = x;
   s.pstride = 0;
+  s.fstride = (long long)g.H * g.W;
   s.ld = g.W;
   s.srcH = g.H;
   s.srcW = g.W;
@@ -92,6 +95,7 @@ RowSrc src_psf(const vmbd_plan* p, const float* f) {
   RowSrc s{};
   s.src = f;
   s.pstride = (long long)g.sf0 * g.sf1;
+  s.fstride = 0;
   s.ld = g.sf1;
   s.srcH = g.sf0;
   s.srcW = g.sf1;
@@ -113,6 +117,7 @@ RowSrc src_residual(const vmbd_plan* p, const float* gres, int exact) {
   RowSrc s{};
   s.src = gres;
   s.pstride = 0;
+  s.fstride = (long long)g.sy0 * g.sy1;
   s.ld = g.sy1;
   s.srcH = g.sy0;
   s.srcW = g.sy1;
@@ -184,9 +189,18 @@ OlaDst dst_adjoint_x(const vmbd_plan* p, float* xbar) {
     if (e__ != hipSuccess) return vmbd_fail_hip(#expr, e__);         \
   } while (0)
 
-int check_plan(const vmbd_plan* p) {
+int check_plan(const vmbd_plan* p, int nframes) {
   if (!p) return vmbd_fail(VMBD_E_ARG, "null plan");
+  if (nframes < 1 || nframes > p->max_frames)
+    return vmbd_fail(VMBD_E_ARG, "nframes=%d outside [1, max_frames=%d]", nframes, p->max_frames);
   return VMBD_OK;
+}
+
+// geometry of one call: all patches of nframes frames
+Geo call_geo(const vmbd_plan* p, int nframes) {
+  Geo g = p->g;
+  g.P = nframes * g.Pf;
+  return g;
 }
 
 }  // namespace
@@ -223,7 +237,8 @@ vmbd_plan* vmbd_plan_create(const vmbd_geom* gm, const float* win_table, int32_t
   g.sf1 = G.sfw;
   g.csf0 = G.csf0;
   g.csf1 = G.csf1;
-  g.P = G.csf0 * G.csf1;
+  g.Pf = G.csf0 * G.csf1;
+  g.P = g.Pf;
   g.sw0 = G.sw0;
   g.sw1 = G.sw1;
   g.hop0 = G.nhop0;
@@ -267,7 +282,11 @@ vmbd_plan* vmbd_plan_create(const vmbd_geom* gm, const float* win_table, int32_t
     vmbd_fail(VMBD_E_ARG, "vmbd_plan_create: patch grid covers %lldx%lld < image %dx%d", sxp0, sxp1, G.H, G.W);
     return nullptr;
   }
-  for (int k = 0; k < g.P; ++k) {
+  if (G.max_frames < 0 || G.max_frames > 65536) {
+    vmbd_fail(VMBD_E_ARG, "vmbd_plan_create: max_frames=%d outside [0, 65536]", G.max_frames);
+    return nullptr;
+  }
+  for (int k = 0; k < g.Pf; ++k) {
     if (win_idx[k] < 0 || win_idx[k] >= n_win) {
       vmbd_fail(VMBD_E_ARG, "vmbd_plan_create: win_idx[%d]=%d outside [0,%d)", k, win_idx[k], n_win);
       return nullptr;
@@ -276,6 +295,7 @@ vmbd_plan* vmbd_plan_create(const vmbd_geom* gm, const float* win_table, int32_t
   vmbd_plan* p = new vmbd_plan();
   p->g = g;
   p->n_win = n_win;
+  p->max_frames = G.max_frames > 0 ? G.max_frames : 1;
   (void)hipGetDevice(&p->device);
   // rows of the OLA gather buffer
   p->mc_fwd = max_cover(g.csf0, g.hop0, g.ext0, g.oc0, g.oc0 + g.sy0);
@@ -291,9 +311,9 @@ vmbd_plan* vmbd_plan_create(const vmbd_geom* gm, const float* win_table, int32_t
     }
   }
   size_t tot = 0;
-  const size_t P = g.P, NB = g.NB;
+  const size_t P = (size_t)g.Pf * p->max_frames, NB = g.NB;
   bool ok = dev_alloc(&p->tw, TW_LEN, tot) && dev_alloc(&p->win, (size_t)n_win * g.sw0 * g.sw1, tot) &&
-            dev_alloc(&p->win_idx, P, tot) && dev_alloc(&p->wsG, P * NB * g.ext0, tot) &&
+            dev_alloc(&p->win_idx, g.Pf, tot) && dev_alloc(&p->wsG, P * NB * g.ext0, tot) &&
             dev_alloc(&p->wsX, P * NB * g.sw0, tot) && dev_alloc(&p->wsF, P * NB * g.sf0, tot) &&
             dev_alloc(&p->Y1, P * NB * g.ext0, tot) && dev_alloc(&p->Y2, P * NB * g.sf0, tot);
   if (!ok) {
@@ -310,7 +330,7 @@ vmbd_plan* vmbd_plan_create(const vmbd_geom* gm, const float* win_table, int32_t
   hipError_t e = hipMemcpy(p->tw, tw.data(), TW_LEN * sizeof(float2), hipMemcpyHostToDevice);
   if (e == hipSuccess)
     e = hipMemcpy(p->win, win_table, (size_t)n_win * g.sw0 * g.sw1 * sizeof(float), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(p->win_idx, win_idx, P * sizeof(int), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(p->win_idx, win_idx, (size_t)g.Pf * sizeof(int), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
     plan_free(p);
     vmbd_fail_hip("vmbd_plan_create: hipMemcpy", e);
@@ -332,73 +352,89 @@ int vmbd_plan_out_shape(const vmbd_plan* plan, int32_t* sy0, int32_t* sy1) {
 
 size_t vmbd_plan_spectrum_elems(const vmbd_plan* plan) {
   if (!plan) return 0;
-  return (size_t)plan->g.P * plan->g.NB * plan->g.Th;
+  return (size_t)plan->g.Pf * plan->g.NB * plan->g.Th;
 }
 
-int vmbd_ola_forward(vmbd_plan* p, const float* x, const float* f, float* y, vmbd_stream_t stream) {
+int vmbd_ola_forward(vmbd_plan* p, int32_t nframes, const float* x, const float* f, float* y,
+                     vmbd_stream_t stream) {
   vmbd_clear_error();
-  if (int r = check_plan(p)) return r;
+  if (int r = check_plan(p, nframes)) return r;
   if (!x || !f || !y) return vmbd_fail(VMBD_E_ARG, "vmbd_ola_forward: null pointer");
   hipStream_t st = (hipStream_t)stream;
-  const Geo& g = p->g;
-  RowSrc sx = src_image(p, x), sf = src_psf(p, f);
-  VMBD_LAUNCH(launch_rows_r2c(g, sx, p->wsX, p->tw, st));
-  VMBD_LAUNCH(launch_rows_r2c(g, sf, p->wsF, p->tw, st));
+  const Geo g = call_geo(p, nframes);
+  p->rows_frames = 0;
+  RowSrc srcs[2] = {src_image(p, x), src_psf(p, f)};
+  float2* outs[2] = {p->wsX, p->wsF};
+  VMBD_LAUNCH(launch_rows_r2c_multi(g, srcs, outs, 2, p->tw, st));
   ColOut o1{p->Y1, g.ext0}, o2{nullptr, 0};
   ColIn none{nullptr, 0, 0, 0};
-  VMBD_LAUNCH(launch_cols(g, COL_FWD, raw_in(p->wsX, sx), raw_in(p->wsF, sf), none, o1, o2, inv_scale(g), nullptr,
-                          p->tw, st));
+  VMBD_LAUNCH(launch_cols(g, COL_FWD, raw_in(p->wsX, srcs[0]), raw_in(p->wsF, srcs[1]), none, o1, o2,
+                          inv_scale(g), nullptr, p->tw, st));
   VMBD_LAUNCH(launch_rows_c2r_ola(g, dst_forward(p, y), p->tw, st));
+  p->rows_frames = nframes;
   return VMBD_OK;
 }
 
-int vmbd_ola_adjoint_x(vmbd_plan* p, const float* gres, const float* f, float* xbar, int32_t exact,
-                       vmbd_stream_t stream) {
+int vmbd_ola_adjoint_x(vmbd_plan* p, int32_t nframes, const float* gres, const float* f, float* xbar,
+                       int32_t exact, vmbd_stream_t stream) {
   vmbd_clear_error();
-  if (int r = check_plan(p)) return r;
+  if (int r = check_plan(p, nframes)) return r;
   if (!gres || !f || !xbar) return vmbd_fail(VMBD_E_ARG, "vmbd_ola_adjoint_x: null pointer");
   hipStream_t st = (hipStream_t)stream;
-  const Geo& g = p->g;
-  RowSrc sg = src_residual(p, gres, exact), sf = src_psf(p, f);
-  VMBD_LAUNCH(launch_rows_r2c(g, sg, p->wsG, p->tw, st));
-  VMBD_LAUNCH(launch_rows_r2c(g, sf, p->wsF, p->tw, st));
+  const Geo g = call_geo(p, nframes);
+  p->rows_frames = 0;
+  RowSrc srcs[2] = {src_residual(p, gres, exact), src_psf(p, f)};
+  float2* outs[2] = {p->wsG, p->wsF};
+  VMBD_LAUNCH(launch_rows_r2c_multi(g, srcs, outs, 2, p->tw, st));
   ColOut o1{p->Y1, g.sw0}, o2{nullptr, 0};
   ColIn none{nullptr, 0, 0, 0};
-  VMBD_LAUNCH(launch_cols(g, COL_ADJ1, raw_in(p->wsG, sg), raw_in(p->wsF, sf), none, o1, o2, inv_scale(g), nullptr,
-                          p->tw, st));
+  VMBD_LAUNCH(launch_cols(g, COL_ADJ1, raw_in(p->wsG, srcs[0]), raw_in(p->wsF, srcs[1]), none, o1, o2,
+                          inv_scale(g), nullptr, p->tw, st));
   VMBD_LAUNCH(launch_rows_c2r_ola(g, dst_adjoint_x(p, xbar), p->tw, st));
   return VMBD_OK;
 }
 
-int vmbd_ola_adjoint_f(vmbd_plan* p, const float* gres, const float* x, float* fbar, int32_t exact,
-                       vmbd_stream_t stream) {
+int vmbd_ola_adjoint_f(vmbd_plan* p, int32_t nframes, const float* gres, const float* x, float* fbar,
+                       int32_t exact, vmbd_stream_t stream) {
   vmbd_clear_error();
-  if (int r = check_plan(p)) return r;
+  if (int r = check_plan(p, nframes)) return r;
   if (!gres || !x || !fbar) return vmbd_fail(VMBD_E_ARG, "vmbd_ola_adjoint_f: null pointer");
   hipStream_t st = (hipStream_t)stream;
-  const Geo& g = p->g;
-  RowSrc sg = src_residual(p, gres, exact), sx = src_image(p, x);
-  VMBD_LAUNCH(launch_rows_r2c(g, sg, p->wsG, p->tw, st));
-  VMBD_LAUNCH(launch_rows_r2c(g, sx, p->wsX, p->tw, st));
+  const Geo g = call_geo(p, nframes);
+  p->rows_frames = 0;
+  RowSrc srcs[2] = {src_residual(p, gres, exact), src_image(p, x)};
+  float2* outs[2] = {p->wsG, p->wsX};
+  VMBD_LAUNCH(launch_rows_r2c_multi(g, srcs, outs, 2, p->tw, st));
   ColOut o1{p->Y2, g.sf0}, o2{nullptr, 0};
   ColIn none{nullptr, 0, 0, 0};
-  VMBD_LAUNCH(launch_cols(g, COL_ADJ1, raw_in(p->wsG, sg), raw_in(p->wsX, sx), none, o1, o2, inv_scale(g), nullptr,
-                          p->tw, st));
+  VMBD_LAUNCH(launch_cols(g, COL_ADJ1, raw_in(p->wsG, srcs[0]), raw_in(p->wsX, srcs[1]), none, o1, o2,
+                          inv_scale(g), nullptr, p->tw, st));
   VMBD_LAUNCH(launch_rows_c2r_stack(g, p->Y2, g.sf0, fbar, g.sf0, g.sf1, p->tw, st));
   return VMBD_OK;
 }
 
-int vmbd_ola_adjoint_xf(vmbd_plan* p, const float* gres, const float* x, const float* f, float* xbar, float* fbar,
-                        int32_t exact, vmbd_stream_t stream) {
+int vmbd_ola_adjoint_xf(vmbd_plan* p, int32_t nframes, const float* gres, const float* x, const float* f,
+                        float* xbar, float* fbar, int32_t exact, int32_t flags, vmbd_stream_t stream) {
   vmbd_clear_error();
-  if (int r = check_plan(p)) return r;
-  if (!gres || !x || !f || !xbar || !fbar) return vmbd_fail(VMBD_E_ARG, "vmbd_ola_adjoint_xf: null pointer");
+  if (int r = check_plan(p, nframes)) return r;
+  if (!gres || !xbar || !fbar) return vmbd_fail(VMBD_E_ARG, "vmbd_ola_adjoint_xf: null pointer");
+  const bool reuse = (flags & VMBD_REUSE_ROWS) != 0;
+  if (reuse && p->rows_frames != nframes)
+    return vmbd_fail(VMBD_E_ARG, "vmbd_ola_adjoint_xf: VMBD_REUSE_ROWS without a preceding forward of %d frames",
+                     nframes);
+  if (!reuse && (!x || !f)) return vmbd_fail(VMBD_E_ARG, "vmbd_ola_adjoint_xf: null x/f");
   hipStream_t st = (hipStream_t)stream;
-  const Geo& g = p->g;
+  const Geo g = call_geo(p, nframes);
   RowSrc sg = src_residual(p, gres, exact), sx = src_image(p, x), sf = src_psf(p, f);
-  VMBD_LAUNCH(launch_rows_r2c(g, sg, p->wsG, p->tw, st));
-  VMBD_LAUNCH(launch_rows_r2c(g, sx, p->wsX, p->tw, st));
-  VMBD_LAUNCH(launch_rows_r2c(g, sf, p->wsF, p->tw, st));
+  if (reuse) {
+    VMBD_LAUNCH(launch_rows_r2c(g, sg, p->wsG, p->tw, st));
+  } else {
+    p->rows_frames = 0;
+    RowSrc srcs[3] = {sg, sx, sf};
+    float2* outs[3] = {p->wsG, p->wsX, p->wsF};
+    VMBD_LAUNCH(launch_rows_r2c_multi(g, srcs, outs, 3, p->tw, st));
+    p->rows_frames = nframes;  // wsX/wsF now hold the rows of this x and f
+  }
   ColOut o1{p->Y1, g.sw0}, o2{p->Y2, g.sf0};
   VMBD_LAUNCH(launch_cols(g, COL_ADJ2, raw_in(p->wsG, sg), raw_in(p->wsF, sf), raw_in(p->wsX, sx), o1, o2,
                           inv_scale(g), nullptr, p->tw, st));
@@ -407,13 +443,14 @@ int vmbd_ola_adjoint_xf(vmbd_plan* p, const float* gres, const float* x, const f
   return VMBD_OK;
 }
 
-int vmbd_spectra(vmbd_plan* p, int32_t kind, const float* z, void* spec, vmbd_stream_t stream) {
+int vmbd_spectra(vmbd_plan* p, int32_t nframes, int32_t kind, const float* z, void* spec, vmbd_stream_t stream) {
   vmbd_clear_error();
-  if (int r = check_plan(p)) return r;
+  if (int r = check_plan(p, nframes)) return r;
   if (!z || !spec) return vmbd_fail(VMBD_E_ARG, "vmbd_spectra: null pointer");
   if (kind != VMBD_CACHE_PSF && kind != VMBD_CACHE_IMAGE) return vmbd_fail(VMBD_E_ARG, "vmbd_spectra: bad kind");
   hipStream_t st = (hipStream_t)stream;
-  const Geo& g = p->g;
+  const Geo g = call_geo(p, nframes);
+  p->rows_frames = 0;
   RowSrc s = kind == VMBD_CACHE_PSF ? src_psf(p, z) : src_image(p, z);
   float2* ws = kind == VMBD_CACHE_PSF ? p->wsF : p->wsX;
   VMBD_LAUNCH(launch_rows_r2c(g, s, ws, p->tw, st));
@@ -423,15 +460,16 @@ int vmbd_spectra(vmbd_plan* p, int32_t kind, const float* z, void* spec, vmbd_st
   return VMBD_OK;
 }
 
-int vmbd_ola_forward_cached(vmbd_plan* p, int32_t kind, const void* spec, const float* u, float* y,
+int vmbd_ola_forward_cached(vmbd_plan* p, int32_t nframes, int32_t kind, const void* spec, const float* u, float* y,
                             vmbd_stream_t stream) {
   vmbd_clear_error();
-  if (int r = check_plan(p)) return r;
+  if (int r = check_plan(p, nframes)) return r;
   if (!spec || !u || !y) return vmbd_fail(VMBD_E_ARG, "vmbd_ola_forward_cached: null pointer");
   if (kind != VMBD_CACHE_PSF && kind != VMBD_CACHE_IMAGE)
     return vmbd_fail(VMBD_E_ARG, "vmbd_ola_forward_cached: bad kind");
   hipStream_t st = (hipStream_t)stream;
-  const Geo& g = p->g;
+  const Geo g = call_geo(p, nframes);
+  p->rows_frames = 0;
   // the free operand is the image for a PSF cache and the PSFs for an image cache
   RowSrc s = kind == VMBD_CACHE_PSF ? src_image(p, u) : src_psf(p, u);
   float2* ws = kind == VMBD_CACHE_PSF ? p->wsX : p->wsF;
@@ -444,15 +482,15 @@ int vmbd_ola_forward_cached(vmbd_plan* p, int32_t kind, const void* spec, const 
   return VMBD_OK;
 }
 
-int vmbd_ola_adjoint_cached(vmbd_plan* p, int32_t kind, const void* spec, const float* gres, float* out,
-                            int32_t exact, vmbd_stream_t stream) {
+int vmbd_ola_adjoint_cached(vmbd_plan* p, int32_t nframes, int32_t kind, const void* spec, const float* gres,
+                            float* out, int32_t exact, vmbd_stream_t stream) {
   vmbd_clear_error();
-  if (int r = check_plan(p)) return r;
+  if (int r = check_plan(p, nframes)) return r;
   if (!spec || !gres || !out) return vmbd_fail(VMBD_E_ARG, "vmbd_ola_adjoint_cached: null pointer");
   if (kind != VMBD_CACHE_PSF && kind != VMBD_CACHE_IMAGE)
     return vmbd_fail(VMBD_E_ARG, "vmbd_ola_adjoint_cached: bad kind");
   hipStream_t st = (hipStream_t)stream;
-  const Geo& g = p->g;
+  const Geo g = call_geo(p, nframes);
   RowSrc sg = src_residual(p, gres, exact);
   VMBD_LAUNCH(launch_rows_r2c(g, sg, p->wsG, p->tw, st));
   ColIn none{nullptr, 0, 0, 0};

@@ -13,7 +13,9 @@ namespace vmbd {
 struct Geo {
   int H, W;
   int sf0, sf1;
-  int csf0, csf1, P;
+  int csf0, csf1;
+  int Pf;  // patches per frame (csf0*csf1)
+  int P;   // patches in this launch (frames * Pf); patch p = frame*Pf + k
   int sw0, sw1;
   int hop0, hop1;
   int Th, Tw, NB;
@@ -29,13 +31,23 @@ struct Geo {
 // [0,srcH) x [0,srcW); multiplied by win[win_idx[p]][r][c] when win != null.
 struct RowSrc {
   const float* src;
-  long long pstride;
+  long long pstride;  // elements between consecutive patches (PSF stacks)
+  long long fstride;  // elements between consecutive frames (images)
   int ld, srcH, srcW;
   int origin;
   int sh0, sh1;
   int rlo, rhi, clo, chi;
   const float* win;
   const int* win_idx;
+};
+
+// Up to three row-transform jobs fused in one launch: blocks [0, blk_end[0])
+// run job 0, [blk_end[0], blk_end[1]) job 1, ...
+struct RowJobs {
+  RowSrc s[3];
+  float2* out[3];
+  int blk_end[3];
+  int n;
 };
 
 // Column-pass input: either raw row-transform output laid out [P][NB][nr]
@@ -74,6 +86,9 @@ enum ColOp { COL_SPEC = 0, COL_FWD = 1, COL_ADJ1 = 2, COL_ADJ2 = 3 };
 
 // Launchers (ola_kernels.hip).  Return hipError_t of the launch.
 hipError_t launch_rows_r2c(const Geo& g, const RowSrc& s, float2* out, const float2* tw, hipStream_t st);
+// several row-transform jobs in one launch (n <= 3)
+hipError_t launch_rows_r2c_multi(const Geo& g, const RowSrc* s, float2* const* out, int n, const float2* tw,
+                                 hipStream_t st);
 hipError_t launch_cols(const Geo& g, int op, const ColIn& a, const ColIn& b, const ColIn& c, const ColOut& o1,
                        const ColOut& o2, float scale, float2* spec_out, const float2* tw, hipStream_t st);
 hipError_t launch_rows_c2r_ola(const Geo& g, const OlaDst& d, const float2* tw, hipStream_t st);

@@ -38,13 +38,27 @@ struct cmax {
 // rows_r2c
 // ----------------------------------------------------------------------------
 template <int TW>
-__global__ __launch_bounds__(WG) void k_rows_r2c(Geo g, RowSrc s, float2* __restrict__ out,
-                                                 const float2* __restrict__ tw) {
+__global__ __launch_bounds__(WG) void k_rows_r2c(Geo g, RowJobs jobs, const float2* __restrict__ tw) {
   constexpr int TPT = TW / 16;
   constexpr int TPW = WG / TPT;
   constexpr int NBN = TW / 2 + 1;
   constexpr int SL = FftLds<TW>::floats2;
   __shared__ float2 sm[cmax<TPW * SL, 2 * TPW * NBN>::v];
+
+  // job of this block (uniform)
+  int ji = 0;
+  int blk0 = 0;
+  if (jobs.n > 1 && (int)blockIdx.x >= jobs.blk_end[0]) {
+    ji = 1;
+    blk0 = jobs.blk_end[0];
+    if (jobs.n > 2 && (int)blockIdx.x >= jobs.blk_end[1]) {
+      ji = 2;
+      blk0 = jobs.blk_end[1];
+    }
+  }
+  const RowSrc s = jobs.s[ji];
+  float2* __restrict__ out = jobs.out[ji];
+  const int blk = blockIdx.x - blk0;
 
   const int tid = threadIdx.x;
   const int tr = tid / TPT;
@@ -52,17 +66,19 @@ __global__ __launch_bounds__(WG) void k_rows_r2c(Geo g, RowSrc s, float2* __rest
   const int nr = s.rhi - s.rlo;
   const int NQ = (nr + 1) >> 1;
   const int nitems = g.P * NQ;
-  const int item = blockIdx.x * TPW + tr;
+  const int item = blk * TPW + tr;
   const bool active = item < nitems;
   const int p = active ? item / NQ : 0;
   const int q = item - p * NQ;
   const int r0 = s.rlo + 2 * q;
-  const int m = p / g.csf1;
-  const int n = p - m * g.csf1;
+  const int fr = p / g.Pf;
+  const int k = p - fr * g.Pf;
+  const int m = k / g.csf1;
+  const int n = k - m * g.csf1;
   const int o0 = (s.origin ? m * g.hop0 : 0) - s.sh0;
   const int o1 = (s.origin ? n * g.hop1 : 0) - s.sh1;
-  const float* base = s.src + (size_t)p * s.pstride;
-  const float* wb = s.win ? s.win + (size_t)s.win_idx[p] * g.sw0 * g.sw1 : nullptr;
+  const float* base = s.src + (size_t)p * s.pstride + (size_t)fr * s.fstride;
+  const float* wb = s.win ? s.win + (size_t)s.win_idx[k] * g.sw0 * g.sw1 : nullptr;
 
   Twiddles<TW> T;
   twiddles_init<TW>(T, t, tw);
@@ -116,7 +132,7 @@ __global__ __launch_bounds__(WG) void k_rows_r2c(Geo g, RowSrc s, float2* __rest
   }
   __syncthreads();
   // bin-major store: consecutive lanes -> consecutive rows of one bin
-  const int item0 = blockIdx.x * TPW;
+  const int item0 = blk * TPW;
   for (int e = tid; e < 2 * TPW * NBN; e += WG) {
     const int c = e / (2 * TPW);
     const int rl = e - c * (2 * TPW);
@@ -130,27 +146,42 @@ __global__ __launch_bounds__(WG) void k_rows_r2c(Geo g, RowSrc s, float2* __rest
 }
 
 template <int TW>
-static hipError_t rows_r2c_t(const Geo& g, const RowSrc& s, float2* out, const float2* tw, hipStream_t st) {
+static hipError_t rows_r2c_t(const Geo& g, const RowSrc* s, float2* const* out, int n, const float2* tw,
+                             hipStream_t st) {
   constexpr int TPW = WG / (TW / 16);
-  const int nr = s.rhi - s.rlo;
-  const long long items = (long long)g.P * ((nr + 1) / 2);
-  const long long blocks = (items + TPW - 1) / TPW;
+  RowJobs jobs{};
+  jobs.n = n;
+  long long blocks = 0;
+  for (int i = 0; i < n; ++i) {
+    jobs.s[i] = s[i];
+    jobs.out[i] = out[i];
+    const int nr = s[i].rhi - s[i].rlo;
+    const long long items = (long long)g.P * ((nr + 1) / 2);
+    blocks += (items + TPW - 1) / TPW;
+    jobs.blk_end[i] = (int)blocks;
+  }
   if (blocks <= 0) return hipSuccess;
-  ProfScope ps("rows_r2c", st);
-  hipLaunchKernelGGL(k_rows_r2c<TW>, dim3((unsigned)blocks), dim3(WG), 0, st, g, s, out, tw);
+  ProfScope ps(n == 1 ? "rows_r2c" : "rows_r2c_multi", st);
+  hipLaunchKernelGGL(k_rows_r2c<TW>, dim3((unsigned)blocks), dim3(WG), 0, st, g, jobs, tw);
   return hipGetLastError();
 }
 
-hipError_t launch_rows_r2c(const Geo& g, const RowSrc& s, float2* out, const float2* tw, hipStream_t st) {
+hipError_t launch_rows_r2c_multi(const Geo& g, const RowSrc* s, float2* const* out, int n, const float2* tw,
+                                 hipStream_t st) {
+  if (n < 1 || n > 3) return hipErrorInvalidValue;
   switch (g.Tw) {
-    case 16: return rows_r2c_t<16>(g, s, out, tw, st);
-    case 32: return rows_r2c_t<32>(g, s, out, tw, st);
-    case 64: return rows_r2c_t<64>(g, s, out, tw, st);
-    case 128: return rows_r2c_t<128>(g, s, out, tw, st);
-    case 256: return rows_r2c_t<256>(g, s, out, tw, st);
-    case 512: return rows_r2c_t<512>(g, s, out, tw, st);
+    case 16: return rows_r2c_t<16>(g, s, out, n, tw, st);
+    case 32: return rows_r2c_t<32>(g, s, out, n, tw, st);
+    case 64: return rows_r2c_t<64>(g, s, out, n, tw, st);
+    case 128: return rows_r2c_t<128>(g, s, out, n, tw, st);
+    case 256: return rows_r2c_t<256>(g, s, out, n, tw, st);
+    case 512: return rows_r2c_t<512>(g, s, out, n, tw, st);
     default: return hipErrorInvalidValue;
   }
+}
+
+hipError_t launch_rows_r2c(const Geo& g, const RowSrc& s, float2* out, const float2* tw, hipStream_t st) {
+  return launch_rows_r2c_multi(g, &s, &out, 1, tw, st);
 }
 
 // ----------------------------------------------------------------------------
@@ -318,8 +349,10 @@ __global__ __launch_bounds__(WG) void k_rows_c2r_ola(Geo g, OlaDst d, const floa
   // single-round launches alias the gather buffer with the FFT slices
   float* rowbuf = d.alias ? (float*)dyn : (float*)(dyn + TPW * SL);
 
-  const int i = blockIdx.x;
+  const int fr = blockIdx.x / d.O0;
+  const int i = blockIdx.x - fr * d.O0;
   const int u = i + d.oc0;
+  const int kf = fr * g.Pf;  // first patch of this frame
   // covering patch rows m: m*hop0 <= u < m*hop0 + ext0
   int mlo = 0, mhi = -1;
   if (u >= 0) {
@@ -345,7 +378,7 @@ __global__ __launch_bounds__(WG) void k_rows_c2r_ola(Geo g, OlaDst d, const floa
     const int m = mlo + mi;
     const int r = u - m * g.hop0;
     const int na = 2 * jp, nb = 2 * jp + 1;
-    const int ka = m * g.csf1 + na;
+    const int ka = kf + m * g.csf1 + na;
     const bool hasb = nb < g.csf1;
     const bool rok = act && r >= 0 && r < d.srow;
     const float2* ra = rok ? d.src + ((size_t)ka * d.srow + r) * g.NB : nullptr;
@@ -355,9 +388,10 @@ __global__ __launch_bounds__(WG) void k_rows_c2r_ola(Geo g, OlaDst d, const floa
     fft<TW, +1>(v, sl, t, T, tw);
     if (d.alias) __syncthreads();  // slices are about to be overwritten by rowbuf
     if (act) {
-      const float* wa = (d.win && r < g.sw0) ? d.win + ((size_t)d.win_idx[ka] * g.sw0 + r) * g.sw1 : nullptr;
+      const float* wa =
+          (d.win && r < g.sw0) ? d.win + ((size_t)d.win_idx[ka - kf] * g.sw0 + r) * g.sw1 : nullptr;
       const float* wbp =
-          (d.win && hasb && r < g.sw0) ? d.win + ((size_t)d.win_idx[ka + 1] * g.sw0 + r) * g.sw1 : nullptr;
+          (d.win && hasb && r < g.sw0) ? d.win + ((size_t)d.win_idx[ka + 1 - kf] * g.sw0 + r) * g.sw1 : nullptr;
       float* ba = rowbuf + (mi * g.csf1 + na) * d.ext1;
       float* bb = rowbuf + (mi * g.csf1 + nb) * d.ext1;
 #pragma unroll
@@ -391,7 +425,7 @@ __global__ __launch_bounds__(WG) void k_rows_c2r_ola(Geo g, OlaDst d, const floa
         acc += sub;
       }
     }
-    d.out[(size_t)i * d.O1 + j] = acc;
+    d.out[((size_t)fr * d.O0 + i) * d.O1 + j] = acc;
   }
 }
 
@@ -416,7 +450,8 @@ template <int TW>
 static hipError_t c2r_ola_t(const Geo& g, const OlaDst& d, const float2* tw, hipStream_t st) {
   const size_t shm = ola_shared_bytes(g, d);
   ProfScope ps(d.win ? "rows_c2r_ola_win" : "rows_c2r_ola", st);
-  hipLaunchKernelGGL(k_rows_c2r_ola<TW>, dim3((unsigned)d.O0), dim3(WG), shm, st, g, d, tw);
+  const int frames = g.P / g.Pf;
+  hipLaunchKernelGGL(k_rows_c2r_ola<TW>, dim3((unsigned)(d.O0 * frames)), dim3(WG), shm, st, g, d, tw);
   return hipGetLastError();
 }
 

@@ -1,15 +1,22 @@
 """Operator plans over torch tensors: the EFF/OLA forward and its transposes
 on the HIP kernels of libvmbd_hip.so.
 
-    plan = OlaPlan(sx, sf, winaux, mode='valid')
+    plan = OlaPlan(sx, sf, winaux, mode='valid', max_frames=1)
     y    = plan.forward(x, f)             # B(x,f) = F x = X f
     xbar = plan.adjoint_x(g, f)           # F' g
     fbar = plan.adjoint_f(g, x)           # X' g
     xbar, fbar = plan.adjoint_xf(g, x, f)
 
-All tensors are float32, contiguous, on the plan's device; results are
-enqueued on the current torch stream.  ``exact=True`` gives the true adjoint,
-``exact=False`` the reference's patch-local transpose (olaGPU.py:276-311).
+Tensors are float32 on the plan's device.  A single frame is x (H, W),
+f (P, sf0, sf1), g/y (sy0, sy1); a batch of B <= max_frames frames adds a
+leading dimension to each and runs all patches of all frames in the same
+launches.  Work is enqueued on the current torch stream.  ``exact=True`` is
+the true adjoint, ``exact=False`` the reference's patch-local transpose
+(olaGPU.py:276-311).
+
+``adjoint_xf`` reuses the row transforms of x and f computed by the
+preceding ``forward`` when it is given the very same tensors (same storage,
+unmodified since, same stream, nothing else run on the plan in between).
 """
 import ctypes
 
@@ -31,7 +38,7 @@ def _as_dev(t, device, name):
 
 
 class OlaPlan:
-    def __init__(self, sx, sf, winaux, mode='valid', tile=None, device=None):
+    def __init__(self, sx, sf, winaux, mode='valid', tile=None, device=None, max_frames=1):
         if mode not in _lib.MODES:
             raise NotImplementedError('Not a valid mode!')
         sx = tuple(int(v) for v in sx)
@@ -45,9 +52,10 @@ class OlaPlan:
         self.P = winaux.csf[0] * winaux.csf[1]
         self.sw = tuple(int(v) for v in winaux.sw)
         self.nhop = tuple(int(v) for v in winaux.nhop)
+        self.max_frames = int(max_frames)
         self.tile = tuple(int(v) for v in (tile if tile is not None else fft_tile(self.sw, sf)))
         g = _lib.VmbdGeom(sx[0], sx[1], sf[0], sf[1], self.csf[0], self.csf[1], self.sw[0], self.sw[1],
-                          self.nhop[0], self.nhop[1], self.tile[0], self.tile[1], _lib.MODES[mode])
+                          self.nhop[0], self.nhop[1], self.tile[0], self.tile[1], _lib.MODES[mode], self.max_frames)
         table = np.ascontiguousarray(winaux.win_table, dtype=np.float32)
         idx = np.ascontiguousarray(winaux.win_idx, dtype=np.int32)
         L = _lib.lib()
@@ -62,6 +70,7 @@ class OlaPlan:
         self.sy = (a.value, b.value)
         self.spec_elems = int(L.vmbd_plan_spectrum_elems(self._h))
         self.workspace_bytes = int(L.vmbd_plan_workspace_bytes(self._h))
+        self._rows_key = None
 
     def __del__(self):
         h = getattr(self, '_h', None)
@@ -74,25 +83,45 @@ class OlaPlan:
             self._h = None
 
     # -- shape checks -------------------------------------------------------
+    def _frames(self, t, base_ndim, name):
+        """Number of frames of t (1 for an unbatched tensor)."""
+        if t.dim() == base_ndim:
+            return 1, False
+        if t.dim() == base_ndim + 1:
+            nf = int(t.shape[0])
+            if not 1 <= nf <= self.max_frames:
+                raise ValueError(f'{name}: {nf} frames, plan allows 1..{self.max_frames}')
+            return nf, True
+        raise IOError('Size missmatch')
+
     def _x(self, x):
         x = _as_dev(x, self.device, 'x')
-        if tuple(x.shape) != self.sx:
+        nf, bat = self._frames(x, 2, 'x')
+        if tuple(x.shape[-2:]) != self.sx:
             raise IOError('Size missmatch between winaux and image size!')
-        return x
+        return x, nf, bat
 
     def _f(self, f):
         f = _as_dev(f, self.device, 'f')
-        if f.dim() != 3 or f.shape[0] != self.P:
+        nf, bat = self._frames(f, 3, 'f')
+        if f.shape[-3] != self.P:
             raise IOError('Size missmatch between winaux and PSF size!')
-        if tuple(f.shape[1:]) != self.sf:
+        if tuple(f.shape[-2:]) != self.sf:
             raise IOError('Size missmatch')
-        return f
+        return f, nf, bat
 
     def _g(self, g):
         g = _as_dev(g, self.device, 'g')
-        if tuple(g.shape) != self.sy:
+        nf, bat = self._frames(g, 2, 'g')
+        if tuple(g.shape[-2:]) != self.sy:
             raise IOError('Sizes incompatible. Expected blurred image!')
-        return g
+        return g, nf, bat
+
+    @staticmethod
+    def _same(*nfs):
+        if len(set(n for n, _ in nfs)) != 1 or len(set(b for _, b in nfs)) != 1:
+            raise ValueError('operands disagree on the number of frames')
+        return nfs[0]
 
     def _empty(self, shape, out):
         if out is None:
@@ -101,72 +130,122 @@ class OlaPlan:
             raise ValueError('out has the wrong shape/dtype/layout')
         return out
 
+    @staticmethod
+    def _lead(nf, bat):
+        return (nf,) if bat else ()
+
+    def _key(self, x, f, nf):
+        # the tensors themselves are held (not their addresses), so a freed
+        # and reallocated buffer can never match a stale key
+        return (x, x._version, f, f._version, nf, torch.cuda.current_stream().cuda_stream)
+
+    def _key_matches(self, x, f, nf):
+        k = self._rows_key
+        if k is None:
+            return False
+        kx, vx, kf, vf, knf, kst = k
+        return (kx is x and kf is f and vx == x._version and vf == f._version and knf == nf
+                and kst == torch.cuda.current_stream().cuda_stream)
+
     # -- operators ----------------------------------------------------------
     def forward(self, x, f, out=None):
-        x, f = self._x(x), self._f(f)
-        y = self._empty(self.sy, out)
-        _lib.check(_lib.lib().vmbd_ola_forward(self._h, _lib.ptr(x), _lib.ptr(f), _lib.ptr(y), _lib.stream_ptr()),
-                   'forward')
+        (x, nx, bx), (f, nfr, bf) = self._x(x), self._f(f)
+        nf, bat = self._same((nx, bx), (nfr, bf))
+        y = self._empty(self._lead(nf, bat) + self.sy, out)
+        self._rows_key = None
+        _lib.check(_lib.lib().vmbd_ola_forward(self._h, nf, _lib.ptr(x), _lib.ptr(f), _lib.ptr(y),
+                                               _lib.stream_ptr()), 'forward')
+        self._rows_key = self._key(x, f, nf)
         return y
 
     def adjoint_x(self, g, f, exact=True, out=None):
-        g, f = self._g(g), self._f(f)
-        xb = self._empty(self.sx, out)
-        _lib.check(_lib.lib().vmbd_ola_adjoint_x(self._h, _lib.ptr(g), _lib.ptr(f), _lib.ptr(xb), int(bool(exact)),
-                                                 _lib.stream_ptr()), 'adjoint_x')
+        (g, ng, bg), (f, nfr, bf) = self._g(g), self._f(f)
+        nf, bat = self._same((ng, bg), (nfr, bf))
+        xb = self._empty(self._lead(nf, bat) + self.sx, out)
+        self._rows_key = None
+        _lib.check(_lib.lib().vmbd_ola_adjoint_x(self._h, nf, _lib.ptr(g), _lib.ptr(f), _lib.ptr(xb),
+                                                 int(bool(exact)), _lib.stream_ptr()), 'adjoint_x')
         return xb
 
     def adjoint_f(self, g, x, exact=True, out=None):
-        g, x = self._g(g), self._x(x)
-        fb = self._empty((self.P,) + self.sf, out)
-        _lib.check(_lib.lib().vmbd_ola_adjoint_f(self._h, _lib.ptr(g), _lib.ptr(x), _lib.ptr(fb), int(bool(exact)),
-                                                 _lib.stream_ptr()), 'adjoint_f')
+        (g, ng, bg), (x, nx, bx) = self._g(g), self._x(x)
+        nf, bat = self._same((ng, bg), (nx, bx))
+        fb = self._empty(self._lead(nf, bat) + (self.P,) + self.sf, out)
+        self._rows_key = None
+        _lib.check(_lib.lib().vmbd_ola_adjoint_f(self._h, nf, _lib.ptr(g), _lib.ptr(x), _lib.ptr(fb),
+                                                 int(bool(exact)), _lib.stream_ptr()), 'adjoint_f')
         return fb
 
-    def adjoint_xf(self, g, x, f, exact=True, out_x=None, out_f=None):
-        g, x, f = self._g(g), self._x(x), self._f(f)
-        xb = self._empty(self.sx, out_x)
-        fb = self._empty((self.P,) + self.sf, out_f)
-        _lib.check(_lib.lib().vmbd_ola_adjoint_xf(self._h, _lib.ptr(g), _lib.ptr(x), _lib.ptr(f), _lib.ptr(xb),
-                                                  _lib.ptr(fb), int(bool(exact)), _lib.stream_ptr()), 'adjoint_xf')
+    def adjoint_xf(self, g, x, f, exact=True, out_x=None, out_f=None, reuse='auto'):
+        (g, ng, bg), (x, nx, bx), (f, nfr, bf) = self._g(g), self._x(x), self._f(f)
+        nf, bat = self._same((ng, bg), (nx, bx), (nfr, bf))
+        xb = self._empty(self._lead(nf, bat) + self.sx, out_x)
+        fb = self._empty(self._lead(nf, bat) + (self.P,) + self.sf, out_f)
+        flags = 0
+        if reuse == 'auto':
+            reuse = self._key_matches(x, f, nf)
+        if reuse:
+            flags |= _lib.REUSE_ROWS
+        if not reuse:
+            self._rows_key = None
+        _lib.check(_lib.lib().vmbd_ola_adjoint_xf(self._h, nf, _lib.ptr(g), _lib.ptr(x), _lib.ptr(f), _lib.ptr(xb),
+                                                  _lib.ptr(fb), int(bool(exact)), flags, _lib.stream_ptr()),
+                   'adjoint_xf')
+        # the workspaces now hold the row transforms of this x and f
+        self._rows_key = self._key(x, f, nf)
         return xb, fb
 
     # -- cached spectra (OlaGPU semantics) ----------------------------------
     def spectra(self, kind, z):
         """kind 'F': z = PSF stack; kind 'X': z = image.  Returns a complex64
-        tensor of shape (P, Tw/2+1, Th)."""
+        tensor of shape ([B,] P, Tw/2+1, Th)."""
         if kind == 'F':
-            z = self._f(z)
+            z, nf, bat = self._f(z)
             k = _lib.CACHE_PSF
         elif kind == 'X':
-            z = self._x(z)
+            z, nf, bat = self._x(z)
             k = _lib.CACHE_IMAGE
         else:
             raise ValueError("kind must be 'F' or 'X'")
-        spec = torch.empty((self.P, self.tile[1] // 2 + 1, self.tile[0]), dtype=torch.complex64, device=self.device)
-        _lib.check(_lib.lib().vmbd_spectra(self._h, k, _lib.ptr(z), _lib.ptr(spec), _lib.stream_ptr()), 'spectra')
+        spec = torch.empty(self._lead(nf, bat) + (self.P, self.tile[1] // 2 + 1, self.tile[0]),
+                           dtype=torch.complex64, device=self.device)
+        self._rows_key = None
+        _lib.check(_lib.lib().vmbd_spectra(self._h, nf, k, _lib.ptr(z), _lib.ptr(spec), _lib.stream_ptr()),
+                   'spectra')
         return spec
+
+    def _spec_frames(self, spec):
+        nf = 1 if spec.dim() == 3 else int(spec.shape[0])
+        if spec.dtype != torch.complex64 or not spec.is_contiguous() or spec.numel() != nf * self.spec_elems:
+            raise ValueError('spectrum tensor does not match the plan')
+        return nf
 
     def forward_cached(self, kind, spec, u, out=None):
         if kind == 'F':
-            u = self._x(u)
+            u, nf, bat = self._x(u)
             k = _lib.CACHE_PSF
         else:
-            u = self._f(u)
+            u, nf, bat = self._f(u)
             k = _lib.CACHE_IMAGE
-        y = self._empty(self.sy, out)
-        _lib.check(_lib.lib().vmbd_ola_forward_cached(self._h, k, _lib.ptr(spec), _lib.ptr(u), _lib.ptr(y),
+        if self._spec_frames(spec) != nf:
+            raise ValueError('spectrum and operand disagree on the number of frames')
+        y = self._empty(self._lead(nf, bat) + self.sy, out)
+        self._rows_key = None
+        _lib.check(_lib.lib().vmbd_ola_forward_cached(self._h, nf, k, _lib.ptr(spec), _lib.ptr(u), _lib.ptr(y),
                                                       _lib.stream_ptr()), 'forward_cached')
         return y
 
     def adjoint_cached(self, kind, spec, g, exact=False, out=None):
-        g = self._g(g)
+        g, nf, bat = self._g(g)
+        if self._spec_frames(spec) != nf:
+            raise ValueError('spectrum and residual disagree on the number of frames')
         if kind == 'F':
             k = _lib.CACHE_PSF
-            o = self._empty(self.sx, out)
+            o = self._empty(self._lead(nf, bat) + self.sx, out)
         else:
             k = _lib.CACHE_IMAGE
-            o = self._empty((self.P,) + self.sf, out)
-        _lib.check(_lib.lib().vmbd_ola_adjoint_cached(self._h, k, _lib.ptr(spec), _lib.ptr(g), _lib.ptr(o),
+            o = self._empty(self._lead(nf, bat) + (self.P,) + self.sf, out)
+        self._rows_key = None
+        _lib.check(_lib.lib().vmbd_ola_adjoint_cached(self._h, nf, k, _lib.ptr(spec), _lib.ptr(g), _lib.ptr(o),
                                                       int(bool(exact)), _lib.stream_ptr()), 'adjoint_cached')
         return o
