@@ -1,0 +1,274 @@
+"""GPU parity of the drop-in modules (gputools, OlaGPU) and of the plain,
+filter and solver kernels against the FP64 oracles."""
+import numpy as np
+import pytest
+
+from oracle import deconv_oracle as DO
+from oracle import filters_oracle as FO
+from oracle import ola_oracle as O
+from tests.helpers import case_inputs, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _t(a, cuda):
+    import torch
+    return torch.as_tensor(np.ascontiguousarray(a, dtype=np.float32), device=cuda)
+
+
+def _n(t):
+    return t.detach().cpu().numpy().astype(np.float64)
+
+
+def _f32(a):
+    return np.float32(a).astype(np.float64)
+
+
+# ------------------------------------------------------------ plain kernels
+def test_plain_chop_pad_ola_crop(cuda):
+    from vmbd_amd import gputools as G
+    from vmbd_amd.imagetools import win2winaux
+    print('case plain chop/pad/ola', flush=True)
+    rng = np.random.default_rng(0)
+    sx, csf = (70, 90), (2, 3)
+    wa = win2winaux(sx, csf, 0.5)
+    x = rng.random(sx)
+    for dtype in ('real', 'complex'):
+        for off in ((0, 0), (3, 2)):
+            got = G.chop_pad_GPU(x, csf, wa.sw, wa.nhop, (64, 64), off, dtype)
+            ref = O.chop_pad(_f32(x), csf, wa.sw, wa.nhop, (64, 64), off)
+            assert np.array_equal(np.real(got.cpu().numpy()), ref.astype(np.float32))
+        got = G.chop_mod_pad_GPU(x, wa.ws_gpu, csf, wa.sw, wa.nhop, (64, 64), dtype=dtype)
+        ref = O.chop_mod_pad(_f32(x), wa.ws.astype(np.float64), csf, wa.sw, wa.nhop, (64, 64))
+        assert rel_l2(np.real(got.cpu().numpy()), ref) < 1e-7
+    st = rng.random((6, 40, 50))
+    got = G.ola_GPU_test(_t(st, cuda), csf, (30, 40), wa.nhop, (2, 5))
+    assert rel_l2(_n(got), O.ola(_f32(st), csf, (30, 40), wa.nhop, (2, 5))) < 1e-7
+    ps = G.pad_stack_GPU(st, (48, 56), (3, 4), 'complex')
+    assert np.array_equal(np.real(ps.cpu().numpy()), O.pad_stack(_f32(st), (48, 56), (3, 4)).astype(np.float32))
+    cs = G.crop_stack_GPU(ps, (40, 50), (3, 4))
+    assert np.array_equal(cs.cpu().numpy(), st.astype(np.float32))
+    c = G.crop_gpu2cpu(_t(x, cuda), (20, 30), (5, 7))
+    assert np.array_equal(c.cpu().numpy(), x.astype(np.float32)[5:25, 7:37])
+    with pytest.raises(IOError):
+        G.crop_stack_GPU(ps, (48, 56), (3, 4))
+
+
+def test_plain_fft2_and_products(cuda):
+    import torch
+    from vmbd_amd import _lib
+    from vmbd_amd import gputools as G
+    print('case plain fft2', flush=True)
+    rng = np.random.default_rng(1)
+    tw = torch.empty(2048, dtype=torch.float32, device=cuda)
+    _lib.call('vmbd_twiddles', _lib.ptr(tw), _lib.stream_ptr())
+    for (B, Th, Tw) in [(3, 16, 16), (2, 64, 128), (5, 256, 256), (1, 512, 32), (2, 1024, 16)]:
+        a = rng.standard_normal((B, Th, Tw)) + 1j * rng.standard_normal((B, Th, Tw))
+        d = torch.as_tensor(a.astype(np.complex64), device=cuda)
+        _lib.call('vmbd_fft2', _lib.ptr(d), B, Th, Tw, -1, 1.0, _lib.ptr(tw), _lib.stream_ptr())
+        assert rel_l2(d.cpu().numpy(), np.fft.fft2(a.astype(np.complex64).astype(np.complex128))) < 2e-6
+        _lib.call('vmbd_fft2', _lib.ptr(d), B, Th, Tw, 1, 1.0 / (Th * Tw), _lib.ptr(tw), _lib.stream_ptr())
+        assert rel_l2(d.cpu().numpy(), a) < 2e-6
+    A = rng.standard_normal((4, 8, 8)) + 1j * rng.standard_normal((4, 8, 8))
+    Bm = rng.standard_normal((8, 8)) + 1j * rng.standard_normal((8, 8))
+    for mode, ref in ((0, A * Bm), (1, A * np.conj(Bm)), (2, np.conj(A) * Bm)):
+        got = G.cmul(A.astype(np.complex64), Bm.astype(np.complex64), mode=mode)
+        assert rel_l2(got.cpu().numpy(), ref) < 1e-6
+    F = A.astype(np.complex64)
+    Y = (A * np.roll(A, 1)).astype(np.complex64)
+    L = (np.abs(Bm) ** 2).astype(np.complex64)
+    got = G.comp_ola_deconv(F, Y, L, 0.3, 0.01)
+    assert rel_l2(got.cpu().numpy(), DO.fourier_div(F.astype(np.complex128), Y, L, 0.3, 0.01)) < 1e-5
+    got = G.comp_ola_gdeconv(F, Y, Y, F, L, 0.2, 0.05)
+    assert rel_l2(got.cpu().numpy(), DO.fourier_div_g(F.astype(np.complex128), Y, Y, F, L, 0.2, 0.05)) < 1e-5
+    got = G.comp_ola_sdeconv(Bm, Bm, F, Y, Y, F, L, 0.5, 4.0, 1e-3)
+    ref = DO.fourier_div_s(Bm.astype(np.complex64), Bm.astype(np.complex64), F, Y, Y, F, L, 0.5, 4.0, 1e-3)
+    assert rel_l2(got.cpu().numpy(), ref) < 1e-5
+
+
+# ------------------------------------------------------------------ filters
+def test_filters(cuda):
+    from vmbd_amd import gputools as G
+    print('case filters', flush=True)
+    rng = np.random.default_rng(2)
+    y = rng.random((67, 53)).astype(np.float32)
+    y[20:40, 10:30] += 0.7
+    yt = _t(y, cuda)
+    gx, gy = G.gradient_gpu(yt)
+    rx, ry = FO.gradient(_f32(y))
+    assert rel_l2(_n(gx), rx) < 1e-6 and rel_l2(_n(gy), ry) < 1e-6
+    gx, gy = G.gradient_gpu(yt, 'same')
+    rx, ry = FO.gradient(_f32(y), 'same')
+    assert rel_l2(_n(gx), rx) < 1e-6 and rel_l2(_n(gy), ry) < 1e-6
+    assert rel_l2(_n(G.laplace_gpu(yt)), FO.laplace(_f32(y))) < 1e-6
+    assert rel_l2(_n(G.laplace_gpu(yt, 'same')), FO.laplace(_f32(y), 'same')) < 1e-6
+    st = rng.random((4, 20, 30)).astype(np.float32)
+    assert rel_l2(_n(G.laplace_stack_gpu(_t(st, cuda))), FO.laplace_stack(_f32(st))) < 1e-6
+    assert rel_l2(_n(G.laplace3d_gpu(_t(st, cuda))), FO.laplace3d(_f32(st))) < 1e-6
+    assert np.array_equal(_n(G.impad_gpu(yt, (19, 20))), FO.impad(_f32(y), (19, 20)))
+    e = yt.clone()
+    G.edgetaper_gpu(e, np.array([12, 17]))
+    assert rel_l2(_n(e), FO.edgetaper(_f32(y), (12, 17))) < 1e-6
+    s = yt.clone()
+    G.shock_filter_gpu(s, iter=3, dt=0.1, h=1)
+    assert rel_l2(_n(s), FO.shock_filter(_f32(y), 3, 0.1, 1.0)) < 1e-5
+    b = yt.clone()
+    G.bfilter_gpu(b, w=5, sigma_d=3, sigma_r=0.1)
+    assert rel_l2(_n(b), FO.bilateral(_f32(y), 5, 3.0, 0.1)) < 1e-5
+    for osz in ((134, 106), (30, 20)):
+        assert rel_l2(_n(G.resize_gpu(yt, osz)), FO.resize(_f32(y), osz)) < 1e-5
+    v = (rng.standard_normal(5000) * 0.5).astype(np.float32)
+    for beta in (1.0, 8.0, 181.0):
+        t = _t(v, cuda)
+        G.modify_sparse23_gpu(t, beta)
+        ref = FO.shrink23(_f32(v), beta)
+        got = _n(t)
+        # a few elements sit on the zero/non-zero switch: compare the rest
+        ok = np.abs(got - ref) < 1e-4 * (1 + np.abs(ref))
+        assert ok.mean() > 0.998, (beta, ok.mean())
+        t = _t(v, cuda)
+        G.modify_sparse_gpu(t, beta, 0.8)
+        ok = np.abs(_n(t) - FO.shrink(_f32(v), beta, 0.8)) < 1e-4 * (1 + np.abs(ref))
+        assert ok.mean() > 0.99
+
+
+# ------------------------------------------------------------------ solver
+def test_solver_primitives(cuda):
+    import torch
+    from vmbd_amd import _lib
+    from vmbd_amd import gputools as G
+    print('case solver primitives', flush=True)
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal(100003).astype(np.float32)
+    t = _t(x, cuda)
+    G.cliplower_GPU(t, 0.1)
+    assert np.array_equal(t.cpu().numpy(), np.maximum(x, np.float32(0.1)))
+    G.clip_GPU(t, 0.2, 0.5)
+    assert np.array_equal(t.cpu().numpy(), np.clip(np.maximum(x, np.float32(0.1)), np.float32(0.2), np.float32(0.5)))
+    assert abs(G.dot(x, x) - float(np.dot(x.astype(np.float64), x))) < 1e-9 * float(np.dot(x, x))
+    f = rng.random((9, 11, 11)).astype(np.float32)
+    f[3] = 0.0
+    fn = G.normalize(_t(f, cuda)).cpu().numpy()
+    ref = f.astype(np.float64).copy()
+    for k in range(9):
+        if ref[k].sum() > 1e-6:
+            ref[k] /= ref[k].sum()
+    assert rel_l2(fn, ref) < 1e-6 and np.all(fn[3] == 0)
+    # PBB step against the formulas of optGPU.py:202-229
+    n = 50000
+    xv = np.maximum(rng.standard_normal(n), 0).astype(np.float32)
+    ox = rng.random(n).astype(np.float32)
+    gv = rng.standard_normal(n).astype(np.float32)
+    og = rng.standard_normal(n).astype(np.float32)
+    X, OX, Gv, OG = (_t(a, cuda) for a in (xv, ox, gv, og))
+    nb = _lib.lib().vmbd_reduce_blocks(n)
+    part = torch.empty(3 * nb, dtype=torch.float64, device=cuda)
+    sums = torch.empty(3, dtype=torch.float64, device=cuda)
+    step = torch.empty(1, dtype=torch.float32, device=cuda)
+    out = torch.empty_like(X)
+    dx = xv.astype(np.float64) - ox
+    dg = gv.astype(np.float64) - og
+    m = (xv == 0) & (gv > 0)
+    dx[m] = 0
+    dg[m] = 0
+    for it in (1, 2):
+        _lib.call('vmbd_pbb_stats', _lib.ptr(X), _lib.ptr(OX), _lib.ptr(Gv), _lib.ptr(OG), n, 0, _lib.ptr(part),
+                  _lib.ptr(sums), _lib.stream_ptr())
+        _lib.call('vmbd_pbb_update', _lib.ptr(out), _lib.ptr(X), _lib.ptr(Gv), n, _lib.ptr(part), it, 0,
+                  _lib.ptr(step), _lib.stream_ptr())
+        s = sums.cpu().numpy()
+        np.testing.assert_allclose(s, [dx @ dx, dx @ dg, dg @ dg], rtol=1e-10)
+        st = (s[0] / (1e-5 + s[1])) if it % 2 == 0 else (s[1] / (1e-5 + s[2]))
+        assert abs(step.item() - st) <= 1e-6 * abs(st)
+        np.testing.assert_allclose(out.cpu().numpy(), np.maximum(xv - gv * np.float32(step.item()), 0),
+                                   rtol=1e-6, atol=1e-6)
+
+
+# ----------------------------------------------------------------- OlaGPU
+@pytest.fixture(scope='module')
+def small_case():
+    sx, csf, sf = (96, 96), (3, 3), (9, 9)
+    x, f, g = case_inputs(40, sx, csf, sf)
+    return sx, csf, sf, x, f, g
+
+
+def test_olagpu_cnv_cnvtp(cuda, small_case):
+    from vmbd_amd.imagetools import win2winaux
+    from vmbd_amd.olaGPU import OlaGPU
+    sx, csf, sf, x, f, g = small_case
+    print('case OlaGPU cnv/cnvtp', flush=True)
+    wa = win2winaux(sx, csf, 0.5)
+    owa = O.Win2WinAux(sx, csf, 0.5)
+    F = OlaGPU(f.astype(np.float32), sx, 'valid', wa)
+    X = OlaGPU(x.astype(np.float32), sf, 'valid', wa)
+    assert F.__id__ == 'F' and X.__id__ == 'X'
+    y = F.cnv(x.astype(np.float32))                      # host in -> host out
+    assert isinstance(y, np.ndarray)
+    y_ref = O.forward(_f32(x), _f32(f), owa)
+    assert rel_l2(y, y_ref) < 1e-5
+    assert rel_l2(_n(X.cnv(_t(f, cuda))), y_ref) < 1e-5   # device in -> device out
+    assert rel_l2(_n(F.cnvtp(g.astype(np.float32))), O.adjoint_x(_f32(g), _f32(f), owa, sx, exact=False)) < 1e-5
+    assert rel_l2(_n(X.cnvtp(_t(g, cuda))), O.adjoint_f(_f32(g), _f32(x), owa, sf, exact=False)) < 1e-5
+    assert rel_l2(_n(F.cnvtp(g.astype(np.float32), exact=True)),
+                  O.adjoint_x(_f32(g), _f32(f), owa, sx, exact=True)) < 1e-5
+    with pytest.raises(IOError):
+        OlaGPU(np.ones((4,) + sf, np.float32), sx, 'valid', wa)
+    with pytest.raises(NotImplementedError):
+        OlaGPU(f.astype(np.float32), sx, 'nope', wa)
+    with pytest.raises(IOError):
+        F.cnv(np.ones((90, 96), np.float32))
+    with pytest.raises(IOError):
+        F.deconv(np.ones((10, 10), np.float32))
+    Ff = OlaGPU(f.astype(np.float32), sx, 'full', wa)
+    with pytest.raises(NotImplementedError):
+        Ff.cnvtp(np.ones(Ff.sy, np.float32))
+
+
+@pytest.mark.parametrize('mode', ['xdirect', 'direct', 'gdirect', 'sparse'])
+def test_olagpu_closed_form_deconv(cuda, small_case, mode):
+    from vmbd_amd.imagetools import win2winaux
+    from vmbd_amd.olaGPU import OlaGPU
+    sx, csf, sf, x, f, g = small_case
+    print(f'case OlaGPU deconv {mode}', flush=True)
+    wa = win2winaux(sx, csf, 0.5)
+    owa = O.Win2WinAux(sx, csf, 0.5)
+    x32, f32 = _f32(x), _f32(f)
+    y = O.forward(x32, f32, owa) + 0.01 * np.random.default_rng(41).standard_normal(O.out_size(sx, sf, 'valid'))
+    y32 = y.astype(np.float32)
+    if mode == 'gdirect':
+        X = OlaGPU(x.astype(np.float32), sf, 'valid', wa)
+        got = X.deconv(y32, mode='gdirect', alpha=0.01, beta=0.01)
+        ref = DO.gdirect(O.OlaOracle(x32, sf, 'valid', owa), _f32(y32), 0.01, 0.01)
+        tol = 1e-4
+    else:
+        F = OlaGPU(f.astype(np.float32), sx, 'valid', wa)
+        Fo = O.OlaOracle(f32, sx, 'valid', owa)
+        if mode == 'xdirect':
+            got = F.deconv(y32, mode='xdirect', alpha=0.05, beta=0.01)
+            ref = DO.xdirect(Fo, _f32(y32), 0.05, 0.01)
+            tol = 1e-4
+        elif mode == 'direct':
+            got = F.deconv(y32, mode='direct', alpha=0.05, beta=0.01)
+            ref = DO.direct(Fo, _f32(y32), 0.05, 0.01)
+            tol = 1e-4
+        else:
+            got = F.deconv(y32, mode='sparse', alpha=0.0001)
+            ref = DO.sparse(Fo, _f32(y32), 0.0001)
+            tol = 2e-3   # thresholded w-steps amplify FP32 rounding
+    assert rel_l2(_n(got), ref) < tol
+
+
+def test_olagpu_lbfgsb_reduces_objective(cuda, small_case):
+    from vmbd_amd.imagetools import win2winaux
+    from vmbd_amd.olaGPU import OlaGPU
+    sx, csf, sf, x, f, g = small_case
+    print('case OlaGPU lbfgsb', flush=True)
+    wa = win2winaux(sx, csf, 0.5)
+    owa = O.Win2WinAux(sx, csf, 0.5)
+    y = O.forward(_f32(x), _f32(f), owa).astype(np.float32)
+    X = OlaGPU(x.astype(np.float32), sf, 'valid', wa)
+    z0 = np.full((9,) + sf, 1.0 / 81, np.float32)
+    obj0 = X.cnvinv_objfun(z0.ravel().astype(np.float64), (9,) + sf, X.plan._g(_t(y, cuda))[0], 0.0, 0.0)
+    fhat, fmin, info = X.deconv(y, z0=z0, mode='lbfgsb', maxfun=30, alpha=0.0, beta=0.0)
+    assert fhat.shape == (9,) + sf and fhat.min() >= 0
+    assert fmin < 0.5 * obj0

@@ -36,7 +36,8 @@ def test_fft_tile():
 
 
 def _header_functions():
-    src = open(os.path.join(ROOT, 'include', 'vmbd.h')).read()
+    src = ''.join(open(os.path.join(ROOT, 'include', f)).read()
+                  for f in sorted(os.listdir(os.path.join(ROOT, 'include'))) if f.endswith('.h'))
     src = re.sub(r'/\*.*?\*/', '', src, flags=re.S)
     return sorted(set(re.findall(r'\b(vmbd_[a-z0-9_]+)\s*\(', src)))
 

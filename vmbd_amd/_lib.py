@@ -49,6 +49,44 @@ SIGNATURES = {
     'vmbd_spectra': (ctypes.c_int, [_P, _I, _I, _F, _P, _P]),
     'vmbd_ola_forward_cached': (ctypes.c_int, [_P, _I, _I, _P, _F, _F, _P]),
     'vmbd_ola_adjoint_cached': (ctypes.c_int, [_P, _I, _I, _P, _F, _F, _I, _P]),
+    # plain (reference-decomposition) kernels, include/vmbd_plain.h
+    'vmbd_pad': (ctypes.c_int, [_F, _I, _I, _F, _I, _I, _I, _I, _I, _P]),
+    'vmbd_chop_pad': (ctypes.c_int, [_F, _I, _I, _F, _I, _I, _I, _I, _I, _I, _I, _I, _I, _I, _F, _F, _I, _P]),
+    'vmbd_pad_stack': (ctypes.c_int, [_F, _I, _I, _I, _F, _I, _I, _I, _I, _I, _P]),
+    'vmbd_crop_stack': (ctypes.c_int, [_F, _I, _I, _I, _F, _I, _I, _I, _I, _I, _P]),
+    'vmbd_crop': (ctypes.c_int, [_F, _I, _I, _F, _I, _I, _I, _I, _I, _P]),
+    'vmbd_ola': (ctypes.c_int, [_F, _F, _I, _I, _I, _I, _I, _I, _I, _I, _I, _I, _I, _P]),
+    'vmbd_fft2': (ctypes.c_int, [_F, _I, _I, _I, _I, ctypes.c_float, _F, _P]),
+    'vmbd_twiddles': (ctypes.c_int, [_F, _P]),
+    'vmbd_cmul': (ctypes.c_int, [_F, _F, _F, ctypes.c_int64, ctypes.c_int64, _I, ctypes.c_float, _P]),
+    'vmbd_fourier_div': (ctypes.c_int, [_F, _F, _F, _F, ctypes.c_int64, ctypes.c_int64, ctypes.c_float,
+                                        ctypes.c_float, _P]),
+    'vmbd_fourier_div_g': (ctypes.c_int, [_F, _F, _F, _F, _F, _F, ctypes.c_int64, ctypes.c_int64, ctypes.c_float,
+                                          ctypes.c_float, _P]),
+    'vmbd_fourier_div_s': (ctypes.c_int, [_F, _F, _F, _F, _F, _F, _F, _F, ctypes.c_int64, ctypes.c_int64,
+                                          ctypes.c_float, ctypes.c_float, ctypes.c_float, _P]),
+    # solver kernels, include/vmbd_solver.h
+    'vmbd_reduce_blocks': (ctypes.c_int, [ctypes.c_int64]),
+    'vmbd_clip': (ctypes.c_int, [_F, ctypes.c_int64, ctypes.c_float, ctypes.c_float, _I, _P]),
+    'vmbd_psf_normalize': (ctypes.c_int, [_F, _I, _I, ctypes.c_float, _P]),
+    'vmbd_dot': (ctypes.c_int, [_F, _F, ctypes.c_int64, _F, _F, _P]),
+    'vmbd_pbb_stats': (ctypes.c_int, [_F, _F, _F, _F, ctypes.c_int64, _I, _F, _F, _P]),
+    'vmbd_pbb_update': (ctypes.c_int, [_F, _F, _F, ctypes.c_int64, _F, _I, _I, _F, _P]),
+    'vmbd_axpby': (ctypes.c_int, [_F, _F, ctypes.c_float, _F, ctypes.c_float, ctypes.c_int64, _P]),
+    'vmbd_mul': (ctypes.c_int, [_F, _F, _F, ctypes.c_int64, _P]),
+    'vmbd_ew': (ctypes.c_int, [_F, _F, _F, ctypes.c_float, ctypes.c_int64, _I, _P]),
+    'vmbd_rl_update': (ctypes.c_int, [_F, _F, _F, _I, _I, ctypes.c_float, _I, _F, _F, _P]),
+    # filters, include/vmbd_filters.h
+    'vmbd_shock_step': (ctypes.c_int, [_F, _F, _I, _I, ctypes.c_float, ctypes.c_float, _P]),
+    'vmbd_bilateral': (ctypes.c_int, [_F, _F, _I, _I, _I, ctypes.c_float, ctypes.c_float, _P]),
+    'vmbd_gradient': (ctypes.c_int, [_F, _F, _F, _I, _I, _I, _P]),
+    'vmbd_impad': (ctypes.c_int, [_F, _F, _I, _I, _I, _I, _P]),
+    'vmbd_laplace': (ctypes.c_int, [_F, _F, _I, _I, _I, _I, _I, _P]),
+    'vmbd_laplace3d': (ctypes.c_int, [_F, _F, _I, _I, _I, _P]),
+    'vmbd_shrink_alpha23': (ctypes.c_int, [_F, ctypes.c_int64, ctypes.c_float, _P]),
+    'vmbd_shrink_alpha': (ctypes.c_int, [_F, ctypes.c_int64, ctypes.c_float, ctypes.c_float, _P]),
+    'vmbd_edgetaper': (ctypes.c_int, [_F, _I, _I, _F, _I, _F, _I, _P]),
+    'vmbd_resize': (ctypes.c_int, [_F, _I, _I, _F, _I, _I, _I, _P]),
     'vmbd_prof_enable': (ctypes.c_int, [_I]),
     'vmbd_prof_reset': (ctypes.c_int, []),
     'vmbd_prof_report': (ctypes.c_int, [ctypes.c_char_p, _I]),
@@ -80,6 +118,11 @@ def check(rc, what=''):
     if rc != VMBD_OK:
         msg = lib().vmbd_last_error().decode(errors='replace')
         raise VmbdError(f'{what}: vmbd error {rc}: {msg}')
+
+
+def call(name, *args):
+    """Call an entry point and raise VmbdError on a non-zero status."""
+    check(getattr(lib(), name)(*args), name)
 
 
 def last_error():

@@ -185,7 +185,10 @@ struct Twiddles {
   float2 w[CNT];
 };
 
-template <int N>
+// LOAD=true reads every twiddle from the table (best when a thread performs
+// several transforms); LOAD=false reads two and builds the powers by complex
+// multiplication (fewer memory instructions for one transform per thread).
+template <int N, bool LOAD = false>
 __device__ __forceinline__ void twiddles_init(Twiddles<N>& T, int t, const float2* __restrict__ tw) {
   constexpr int TPT = N / 16;
   constexpr int R1 = Plan<N>::R1;
@@ -195,10 +198,15 @@ __device__ __forceinline__ void twiddles_init(Twiddles<N>& T, int t, const float
 #pragma unroll
     for (int b = 0; b < NB1; ++b) {
       const int k = (t + b * TPT) & 15;
-      float2 p[R1];
-      twiddle_powers<R1>(tw[k * STR], tw[(4 * k * STR) & (TW_LEN - 1)], p);
+      if constexpr (LOAD) {
 #pragma unroll
-      for (int r = 1; r < R1; ++r) T.w[b * (R1 - 1) + r - 1] = p[r];
+        for (int r = 1; r < R1; ++r) T.w[b * (R1 - 1) + r - 1] = tw[(k * r * STR) & (TW_LEN - 1)];
+      } else {
+        float2 p[R1];
+        twiddle_powers<R1>(tw[k * STR], tw[(4 * k * STR) & (TW_LEN - 1)], p);
+#pragma unroll
+        for (int r = 1; r < R1; ++r) T.w[b * (R1 - 1) + r - 1] = p[r];
+      }
     }
   }
 }

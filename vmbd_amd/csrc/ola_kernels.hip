@@ -45,7 +45,7 @@ __global__ __launch_bounds__(WG) void k_rows_r2c(Geo g, RowJobs jobs, const floa
   constexpr int SL = FftLds<TW>::floats2;
   __shared__ float2 sm[cmax<TPW * SL, 2 * TPW * NBN>::v];
 
-  // job of this block (uniform)
+  // grid: x = row-pair blocks of all jobs (job-major), y = patch (frame-major)
   int ji = 0;
   int blk0 = 0;
   if (jobs.n > 1 && (int)blockIdx.x >= jobs.blk_end[0]) {
@@ -58,18 +58,16 @@ __global__ __launch_bounds__(WG) void k_rows_r2c(Geo g, RowJobs jobs, const floa
   }
   const RowSrc s = jobs.s[ji];
   float2* __restrict__ out = jobs.out[ji];
-  const int blk = blockIdx.x - blk0;
+  const int q0 = ((int)blockIdx.x - blk0) * TPW;  // first row pair of the block
 
   const int tid = threadIdx.x;
   const int tr = tid / TPT;
   const int t = tid - tr * TPT;
   const int nr = s.rhi - s.rlo;
   const int NQ = (nr + 1) >> 1;
-  const int nitems = g.P * NQ;
-  const int item = blk * TPW + tr;
-  const bool active = item < nitems;
-  const int p = active ? item / NQ : 0;
-  const int q = item - p * NQ;
+  const int q = q0 + tr;
+  const bool active = q < NQ;
+  const int p = blockIdx.y;
   const int r0 = s.rlo + 2 * q;
   const int fr = p / g.Pf;
   const int k = p - fr * g.Pf;
@@ -131,16 +129,19 @@ __global__ __launch_bounds__(WG) void k_rows_r2c(Geo g, RowJobs jobs, const floa
     }
   }
   __syncthreads();
-  // bin-major store: consecutive lanes -> consecutive rows of one bin
-  const int item0 = blk * TPW;
-  for (int e = tid; e < 2 * TPW * NBN; e += WG) {
-    const int c = e / (2 * TPW);
-    const int rl = e - c * (2 * TPW);
-    const int it = item0 + (rl >> 1);
-    if (it < nitems) {
-      const int pp = it / NQ;
-      const int rr = 2 * (it - pp * NQ) + (rl & 1);
-      if (rr < nr) out[((size_t)pp * NBN + c) * nr + rr] = sm[rl * NBN + c];
+  // bin-major store: consecutive lanes -> consecutive rows of one bin; the
+  // block's rows are the contiguous range [2*q0, 2*q0 + 2*TPW) of patch p
+  float2* dst = out + (size_t)p * NBN * nr + 2 * q0;
+  const int nrow = min(2 * TPW, nr - 2 * q0);
+  if constexpr (2 * TPW <= WG) {
+    const int rl = tid & (2 * TPW - 1);
+    if (rl < nrow)
+      for (int c = tid / (2 * TPW); c < NBN; c += WG / (2 * TPW)) dst[(size_t)c * nr + rl] = sm[rl * NBN + c];
+  } else {
+    for (int e = tid; e < 2 * TPW * NBN; e += WG) {
+      const int c = e / (2 * TPW);
+      const int rl = e - c * (2 * TPW);
+      if (rl < nrow) dst[(size_t)c * nr + rl] = sm[rl * NBN + c];
     }
   }
 }
@@ -151,18 +152,17 @@ static hipError_t rows_r2c_t(const Geo& g, const RowSrc* s, float2* const* out, 
   constexpr int TPW = WG / (TW / 16);
   RowJobs jobs{};
   jobs.n = n;
-  long long blocks = 0;
+  int blocks = 0;
   for (int i = 0; i < n; ++i) {
     jobs.s[i] = s[i];
     jobs.out[i] = out[i];
-    const int nr = s[i].rhi - s[i].rlo;
-    const long long items = (long long)g.P * ((nr + 1) / 2);
-    blocks += (items + TPW - 1) / TPW;
-    jobs.blk_end[i] = (int)blocks;
+    const int nq = (s[i].rhi - s[i].rlo + 1) / 2;
+    blocks += (nq + TPW - 1) / TPW;
+    jobs.blk_end[i] = blocks;
   }
-  if (blocks <= 0) return hipSuccess;
+  if (blocks <= 0 || g.P <= 0) return hipSuccess;
   ProfScope ps(n == 1 ? "rows_r2c" : "rows_r2c_multi", st);
-  hipLaunchKernelGGL(k_rows_r2c<TW>, dim3((unsigned)blocks), dim3(WG), 0, st, g, jobs, tw);
+  hipLaunchKernelGGL(k_rows_r2c<TW>, dim3((unsigned)blocks, (unsigned)g.P), dim3(WG), 0, st, g, jobs, tw);
   return hipGetLastError();
 }
 
@@ -218,11 +218,10 @@ __device__ __forceinline__ void col_store_rows(const float2 (&v)[16], float2* st
   }
   __syncthreads();
   const int c0 = strip * CS;
-  const int ncol = min(CS, NB - c0);
-  for (int e = threadIdx.x; e < o.nrows * CS; e += WG) {
-    const int r = e / CS;
-    const int ci = e - r * CS;
-    if (ci < ncol) o.p[((size_t)p * o.nrows + r) * NB + c0 + ci] = st[r * (CS + 1) + ci];
+  const int ci = threadIdx.x & (CS - 1);
+  if (c0 + ci < NB) {
+    float2* dst = o.p + (size_t)p * o.nrows * NB + c0 + ci;
+    for (int r = threadIdx.x / CS; r < o.nrows; r += WG / CS) dst[(size_t)r * NB] = st[r * (CS + 1) + ci];
   }
   __syncthreads();  // staging drained before the slices are reused
 }
@@ -236,9 +235,8 @@ __global__ __launch_bounds__(WG) void k_cols(Geo g, ColIn a, ColIn b, ColIn c3, 
   __shared__ float2 sm[(OP == COL_SPEC) ? CS * SL : cmax<CS * SL, TH * (CS + 1)>::v];
 
   const int NB = g.NB;
-  const int strips = (NB + CS - 1) / CS;
-  const int p = blockIdx.x / strips;
-  const int strip = blockIdx.x - p * strips;
+  const int p = blockIdx.y;
+  const int strip = blockIdx.x;
   const int cc = threadIdx.x / TPT;
   const int t = threadIdx.x - cc * TPT;
   const int c = strip * CS + cc;
@@ -246,7 +244,7 @@ __global__ __launch_bounds__(WG) void k_cols(Geo g, ColIn a, ColIn b, ColIn c3, 
   float2* sl = sm + cc * SL;
 
   Twiddles<TH> T;
-  twiddles_init<TH>(T, t, tw);
+  twiddles_init<TH, true>(T, t, tw);
 
   float2 A[16];
   col_load<TH>(A, a, p, c, NB, t, active);
@@ -282,7 +280,7 @@ static hipError_t cols_t(const Geo& g, int op, const ColIn& a, const ColIn& b, c
                          const ColOut& o2, float scale, float2* spec_out, const float2* tw, hipStream_t st) {
   constexpr int CS = WG / (TH / 16);
   const int strips = (g.NB + CS - 1) / CS;
-  const dim3 grid((unsigned)(g.P * strips));
+  const dim3 grid((unsigned)strips, (unsigned)g.P);
   static const char* names[4] = {"cols_spec", "cols_fwd", "cols_adj1", "cols_adj2"};
   ProfScope ps(names[op & 3], st);
   switch (op) {
@@ -323,16 +321,28 @@ hipError_t launch_cols(const Geo& g, int op, const ColIn& a, const ColIn& b, con
 // Z[c] = A[c] + i B[c] over the full length from two half spectra (Hermitian
 // extension), in the input mapping of the transform.
 template <int TW>
-__device__ __forceinline__ void c2r_load(float2 (&v)[16], const float2* ra, const float2* rb, int t) {
+__device__ __forceinline__ void c2r_load(float2 (&v)[16], const float2* ra, const float2* rb, bool hasb, int t) {
+  // ra, rb: valid rows (rb == ra when there is no second row, hasb false)
   constexpr int TPT = TW / 16;
+  const float sb = hasb ? 1.f : 0.f;
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
     const int c = t + TPT * r;
-    const bool lo = c <= TW / 2;
-    const int cc = lo ? c : TW - c;
-    const float2 A = ra ? ra[cc] : make_float2(0.f, 0.f);
-    const float2 B = rb ? rb[cc] : make_float2(0.f, 0.f);
-    v[r] = lo ? make_float2(A.x - B.y, A.y + B.x) : make_float2(A.x + B.y, B.x - A.y);
+    if (TPT * r + TPT - 1 <= TW / 2) {  // whole slot in the stored half (folded after unrolling)
+      const float2 A = ra[c];
+      const float2 B = cscale(rb[c], sb);
+      v[r] = make_float2(A.x - B.y, A.y + B.x);
+    } else if (TPT * r > TW / 2) {      // whole slot mirrored
+      const float2 A = ra[TW - c];
+      const float2 B = cscale(rb[TW - c], sb);
+      v[r] = make_float2(A.x + B.y, B.x - A.y);
+    } else {
+      const bool lo = c <= TW / 2;
+      const int cc = lo ? c : TW - c;
+      const float2 A = ra[cc];
+      const float2 B = cscale(rb[cc], sb);
+      v[r] = lo ? make_float2(A.x - B.y, A.y + B.x) : make_float2(A.x + B.y, B.x - A.y);
+    }
   }
 }
 
@@ -349,8 +359,8 @@ __global__ __launch_bounds__(WG) void k_rows_c2r_ola(Geo g, OlaDst d, const floa
   // single-round launches alias the gather buffer with the FFT slices
   float* rowbuf = d.alias ? (float*)dyn : (float*)(dyn + TPW * SL);
 
-  const int fr = blockIdx.x / d.O0;
-  const int i = blockIdx.x - fr * d.O0;
+  const int fr = blockIdx.y;
+  const int i = blockIdx.x;
   const int u = i + d.oc0;
   const int kf = fr * g.Pf;  // first patch of this frame
   // covering patch rows m: m*hop0 <= u < m*hop0 + ext0
@@ -381,17 +391,17 @@ __global__ __launch_bounds__(WG) void k_rows_c2r_ola(Geo g, OlaDst d, const floa
     const int ka = kf + m * g.csf1 + na;
     const bool hasb = nb < g.csf1;
     const bool rok = act && r >= 0 && r < d.srow;
-    const float2* ra = rok ? d.src + ((size_t)ka * d.srow + r) * g.NB : nullptr;
-    const float2* rb = (rok && hasb) ? d.src + ((size_t)(ka + 1) * d.srow + r) * g.NB : nullptr;
+    const float2* ra = rok ? d.src + ((size_t)ka * d.srow + r) * g.NB : d.src;
+    const float2* rb = (rok && hasb) ? ra + (size_t)d.srow * g.NB : ra;
+    // window rows of both patches (addresses resolved before the transform)
+    const bool wrow = d.win && rok && r < g.sw0;
+    const float* wa = wrow ? d.win + ((size_t)d.win_idx[ka - kf] * g.sw0 + r) * g.sw1 : nullptr;
+    const float* wbp = (wrow && hasb) ? d.win + ((size_t)d.win_idx[ka + 1 - kf] * g.sw0 + r) * g.sw1 : wa;
     float2 v[16];
-    c2r_load<TW>(v, ra, rb, t);
+    c2r_load<TW>(v, ra, rb, rok && hasb, t);
     fft<TW, +1>(v, sl, t, T, tw);
     if (d.alias) __syncthreads();  // slices are about to be overwritten by rowbuf
     if (act) {
-      const float* wa =
-          (d.win && r < g.sw0) ? d.win + ((size_t)d.win_idx[ka - kf] * g.sw0 + r) * g.sw1 : nullptr;
-      const float* wbp =
-          (d.win && hasb && r < g.sw0) ? d.win + ((size_t)d.win_idx[ka + 1 - kf] * g.sw0 + r) * g.sw1 : nullptr;
       float* ba = rowbuf + (mi * g.csf1 + na) * d.ext1;
       float* bb = rowbuf + (mi * g.csf1 + nb) * d.ext1;
 #pragma unroll
@@ -400,8 +410,9 @@ __global__ __launch_bounds__(WG) void k_rows_c2r_ola(Geo g, OlaDst d, const floa
         if (col < d.ext1) {
           float xa = v[mm].x, xb = v[mm].y;
           if (d.win) {
-            xa = (wa && col < g.sw1) ? xa * wa[col] : 0.f;
-            xb = (wbp && col < g.sw1) ? xb * wbp[col] : 0.f;
+            const bool okc = wrow && col < g.sw1;
+            xa = okc ? xa * wa[col] : 0.f;
+            xb = okc ? xb * wbp[col] : 0.f;
           }
           ba[col] = xa;
           if (hasb) bb[col] = xb;
@@ -411,17 +422,21 @@ __global__ __launch_bounds__(WG) void k_rows_c2r_ola(Geo g, OlaDst d, const floa
   }
   __syncthreads();
   // fixed-order gather over covering patches
+  const float inv_hop1 = 1.0f / (float)g.hop1;
   for (int j = threadIdx.x; j < d.O1; j += WG) {
     const int vcol = j + d.oc1;
     float acc = 0.f;
     if (vcol >= 0) {
-      const int nhi = min(g.csf1 - 1, vcol / g.hop1);
+      // nhi = min(csf1-1, floor(vcol/hop1)) via a float reciprocal + exact fix-up
+      int qh = (int)((float)vcol * inv_hop1);
+      qh += ((qh + 1) * g.hop1 <= vcol) - (qh * g.hop1 > vcol);
+      const int nhi = min(g.csf1 - 1, qh);
+      const int lc0 = vcol - nhi * g.hop1;
       for (int mi = 0; mi < Mc; ++mi) {
         const float* rbm = rowbuf + mi * g.csf1 * d.ext1;
         // covering n are nhi, nhi-1, ... while the local column stays < ext1
         float sub = 0.f;
-        for (int nn = nhi, lc = vcol - nhi * g.hop1; nn >= 0 && lc < d.ext1; --nn, lc += g.hop1)
-          sub += rbm[nn * d.ext1 + lc];
+        for (int nn = nhi, lc = lc0; nn >= 0 && lc < d.ext1; --nn, lc += g.hop1) sub += rbm[nn * d.ext1 + lc];
         acc += sub;
       }
     }
@@ -451,7 +466,7 @@ static hipError_t c2r_ola_t(const Geo& g, const OlaDst& d, const float2* tw, hip
   const size_t shm = ola_shared_bytes(g, d);
   ProfScope ps(d.win ? "rows_c2r_ola_win" : "rows_c2r_ola", st);
   const int frames = g.P / g.Pf;
-  hipLaunchKernelGGL(k_rows_c2r_ola<TW>, dim3((unsigned)(d.O0 * frames)), dim3(WG), shm, st, g, d, tw);
+  hipLaunchKernelGGL(k_rows_c2r_ola<TW>, dim3((unsigned)d.O0, (unsigned)frames), dim3(WG), shm, st, g, d, tw);
   return hipGetLastError();
 }
 
@@ -482,17 +497,17 @@ __global__ __launch_bounds__(WG) void k_rows_c2r_stack(Geo g, const float2* __re
   const int tr = threadIdx.x / TPT;
   const int t = threadIdx.x - tr * TPT;
   const int NQ = (nrows + 1) >> 1;
-  const int item = blockIdx.x * TPW + tr;
-  const bool act = item < g.P * NQ;
-  const int p = act ? item / NQ : 0;
-  const int q = item - p * NQ;
+  const int q = blockIdx.x * TPW + tr;
+  const bool act = q < NQ;
+  const int p = blockIdx.y;
   const int r0 = 2 * q, r1 = 2 * q + 1;
-  const float2* ra = act ? src + ((size_t)p * srow + r0) * g.NB : nullptr;
-  const float2* rb = (act && r1 < nrows) ? src + ((size_t)p * srow + r1) * g.NB : nullptr;
+  const float2* ra = act ? src + ((size_t)p * srow + r0) * g.NB : src;
+  const bool hasb = act && r1 < nrows;
+  const float2* rb = hasb ? ra + g.NB : ra;
   Twiddles<TW> T;
   twiddles_init<TW>(T, t, tw);
   float2 v[16];
-  c2r_load<TW>(v, ra, rb, t);
+  c2r_load<TW>(v, ra, rb, hasb, t);
   float2* sl = fl + tr * SL;
   fft<TW, +1>(v, sl, t, T, tw);
   if (act) {
@@ -511,12 +526,11 @@ template <int TW>
 static hipError_t c2r_stack_t(const Geo& g, const float2* src, int srow, float* out, int nrows, int ncols,
                               const float2* tw, hipStream_t st) {
   constexpr int TPW = WG / (TW / 16);
-  const long long items = (long long)g.P * ((nrows + 1) / 2);
-  const long long blocks = (items + TPW - 1) / TPW;
-  if (blocks <= 0) return hipSuccess;
+  const int blocks = ((nrows + 1) / 2 + TPW - 1) / TPW;
+  if (blocks <= 0 || g.P <= 0) return hipSuccess;
   ProfScope ps("rows_c2r_stack", st);
-  hipLaunchKernelGGL(k_rows_c2r_stack<TW>, dim3((unsigned)blocks), dim3(WG), 0, st, g, src, srow, out, nrows,
-                     ncols, tw);
+  hipLaunchKernelGGL(k_rows_c2r_stack<TW>, dim3((unsigned)blocks, (unsigned)g.P), dim3(WG), 0, st, g, src, srow,
+                     out, nrows, ncols, tw);
   return hipGetLastError();
 }
 

@@ -1,0 +1,362 @@
+// Solver kernels for gfx950: projections, deterministic reductions and the
+// projected Barzilai-Borwein step with its step length computed on the device.
+//
+// Reductions are two-stage with a fixed grid: every block writes one partial
+// per quantity, and the consumer reduces the partials in a fixed order, so
+// results are bitwise reproducible and no host synchronisation is needed
+// (the iteration can be captured in a HIP graph).  Block sums use wavefront
+// shuffles (DPP/permute through __shfl_xor) and one LDS exchange per wave.
+#include <hip/hip_runtime.h>
+
+#include "../../include/vmbd_solver.h"
+#include "capi_util.h"
+#include "prof.h"
+
+namespace vmbd {
+namespace {
+
+constexpr int TB = 256;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// block sum of up to 3 values (double accumulation), result valid in thread 0
+template <int K>
+__device__ __forceinline__ void block_sum(double (&v)[K]) {
+  __shared__ double sh[K][TB / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    v[k] = wave_sum(v[k]);
+    if (lane == 0) sh[k][w] = v[k];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      double s = 0.0;
+      for (int i = 0; i < TB / 64; ++i) s += sh[k][i];
+      v[k] = s;
+    }
+  }
+  __syncthreads();
+}
+
+// sum of K partial arrays (layout [K][nb]) in fixed order, broadcast to the block
+template <int K>
+__device__ __forceinline__ void reduce_partials(const double* partials, int nb, double (&out)[K]) {
+  double v[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nb; i += TB) s += partials[k * nb + i];
+    v[k] = s;
+  }
+  block_sum<K>(v);
+  __shared__ double bc[K];
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < K; ++k) bc[k] = v[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; ++k) out[k] = bc[k];
+}
+
+__global__ __launch_bounds__(TB) void k_clip(float* x, long long n, float lb, float ub, int flags) {
+  for (long long e = (long long)blockIdx.x * TB + threadIdx.x; e < n; e += (long long)gridDim.x * TB) {
+    float v = x[e];
+    if (flags & VMBD_CLIP_LOWER) v = v < lb ? lb : v;
+    if (flags & VMBD_CLIP_UPPER) v = v > ub ? ub : v;
+    x[e] = v;
+  }
+}
+
+// one block per slice: f_k /= sum(f_k) when the sum exceeds eps
+__global__ __launch_bounds__(TB) void k_normalize(float* f, int len, float eps) {
+  float* s = f + (long long)blockIdx.x * len;
+  double v[1] = {0.0};
+  for (int i = threadIdx.x; i < len; i += TB) v[0] += (double)s[i];
+  block_sum<1>(v);
+  __shared__ float tot;
+  if (threadIdx.x == 0) tot = (float)v[0];
+  __syncthreads();
+  const float t = tot;
+  if (t > eps)
+    for (int i = threadIdx.x; i < len; i += TB) s[i] /= t;
+}
+
+// partials[0][b] = sum a*b over the block's share; (a == b allowed)
+__global__ __launch_bounds__(TB) void k_dot_partials(const float* a, const float* b, long long n, double* partials) {
+  double v[1] = {0.0};
+  for (long long e = (long long)blockIdx.x * TB + threadIdx.x; e < n; e += (long long)gridDim.x * TB)
+    v[0] += (double)a[e] * (double)b[e];
+  block_sum<1>(v);
+  if (threadIdx.x == 0) partials[blockIdx.x] = v[0];
+}
+
+__global__ __launch_bounds__(TB) void k_reduce_out(const double* partials, int nb, int K, double* out) {
+  if (K == 1) {
+    double r[1];
+    reduce_partials<1>(partials, nb, r);
+    if (threadIdx.x == 0) out[0] = r[0];
+  } else {
+    double r[3];
+    reduce_partials<3>(partials, nb, r);
+    if (threadIdx.x == 0) {
+      out[0] = r[0];
+      out[1] = r[1];
+      out[2] = r[2];
+    }
+  }
+}
+
+// PBB, stage 1 (optGPU.py:202-207 + 222-225): dx = x - oldx, dg = g - oldg,
+// zeroed where x == 0 and g > 0 (clip2bound, optGPU.py:129-132) unless
+// unconstrained; partials of sum dx^2, sum dx dg, sum dg^2.
+__global__ __launch_bounds__(TB) void k_pbb_stats(const float* x, const float* oldx, const float* g,
+                                                  const float* oldg, long long n, int unconstrained,
+                                                  double* partials) {
+  double v[3] = {0.0, 0.0, 0.0};
+  for (long long e = (long long)blockIdx.x * TB + threadIdx.x; e < n; e += (long long)gridDim.x * TB) {
+    const float xv = x[e], gv = g[e];
+    float dx = xv - oldx[e];
+    float dg = gv - oldg[e];
+    if (!unconstrained && xv == 0.f && gv > 0.f) {
+      dx = 0.f;
+      dg = 0.f;
+    }
+    v[0] += (double)dx * dx;
+    v[1] += (double)dx * dg;
+    v[2] += (double)dg * dg;
+  }
+  block_sum<3>(v);
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x] = v[0];
+    partials[gridDim.x + blockIdx.x] = v[1];
+    partials[2 * gridDim.x + blockIdx.x] = v[2];
+  }
+}
+
+// PBB, stage 2 (optGPU.py:222-229): BB1 step on even iterations,
+// sum dx^2/(1e-5 + sum dx dg); BB2 on odd, sum dx dg/(1e-5 + sum dg^2);
+// xnew = x - step g, projected onto x >= 0 unless unconstrained.  Every block
+// reduces the stage-1 partials itself (same order -> same step everywhere).
+__global__ __launch_bounds__(TB) void k_pbb_update(float* xnew, const float* x, const float* g, long long n,
+                                                   const double* partials, int nb, int iter, int unconstrained,
+                                                   float* step_out) {
+  double s[3];
+  reduce_partials<3>(partials, nb, s);
+  const float sxx = (float)s[0], sxg = (float)s[1], sgg = (float)s[2];
+  const float step = (iter % 2 == 0) ? sxx / (1e-5f + sxg) : sxg / (1e-5f + sgg);
+  if (blockIdx.x == 0 && threadIdx.x == 0 && step_out) *step_out = step;
+  for (long long e = (long long)blockIdx.x * TB + threadIdx.x; e < n; e += (long long)gridDim.x * TB) {
+    float v = x[e] - g[e] * step;
+    if (!unconstrained) v = v < 0.f ? 0.f : v;
+    xnew[e] = v;
+  }
+}
+
+// out = a*s + b*t (residuals, Tikhonov terms)
+__global__ __launch_bounds__(TB) void k_axpby(float* out, const float* a, float s, const float* b, float t,
+                                              long long n) {
+  for (long long e = (long long)blockIdx.x * TB + threadIdx.x; e < n; e += (long long)gridDim.x * TB)
+    out[e] = a[e] * s + (b ? b[e] * t : 0.f);
+}
+
+// elementwise family of vmbd_ew
+__global__ __launch_bounds__(TB) void k_ew(float* out, const float* a, const float* b, float s, long long n, int op) {
+  for (long long e = (long long)blockIdx.x * TB + threadIdx.x; e < n; e += (long long)gridDim.x * TB) {
+    const float x = a[e];
+    float r;
+    switch (op) {
+      case VMBD_EW_MUL: r = x * b[e]; break;
+      case VMBD_EW_DIV: r = x / b[e]; break;
+      case VMBD_EW_SDIV: r = s / x; break;
+      case VMBD_EW_SUB: r = x - b[e]; break;
+      case VMBD_EW_AXPY: r = x + s * b[e]; break;
+      default: r = s * x; break;
+    }
+    out[e] = r;
+  }
+}
+
+// out = a * b (masks)
+__global__ __launch_bounds__(TB) void k_mul(float* out, const float* a, const float* b, long long n) {
+  for (long long e = (long long)blockIdx.x * TB + threadIdx.x; e < n; e += (long long)gridDim.x * TB)
+    out[e] = a[e] * b[e];
+}
+
+// Multiplicative x-step of the online loop (vmbd_chimney.py:143-147), stage 1:
+// x *= max(num/(den + tol), tol); block maxima of x over the interior
+// rows/cols [border, size-border).
+__global__ __launch_bounds__(TB) void k_rl_step(float* x, const float* num, const float* den, int H, int W, float tol,
+                                                int border, float* partial_max) {
+  float mx = -INFINITY;
+  const long long n = (long long)H * W;
+  for (long long e = (long long)blockIdx.x * TB + threadIdx.x; e < n; e += (long long)gridDim.x * TB) {
+    float fac = num[e] / (den[e] + tol);
+    fac = fac < tol ? tol : fac;
+    const float v = x[e] * fac;
+    x[e] = v;
+    const int i = (int)(e / W), j = (int)(e - (long long)(e / W) * W);
+    if (i >= border && i < H - border && j >= border && j < W - border) mx = fmaxf(mx, v);
+  }
+  mx = wave_max(mx);
+  __shared__ float sh[TB / 64];
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float m = sh[0];
+    for (int i = 1; i < TB / 64; ++i) m = fmaxf(m, sh[i]);
+    partial_max[blockIdx.x] = m;
+  }
+}
+
+// stage 2: x = min(x, max over partials); the maximum is also stored
+__global__ __launch_bounds__(TB) void k_rl_clip(float* x, long long n, const float* partial_max, int nb,
+                                                float* xmax_out) {
+  float m = -INFINITY;
+  for (int i = threadIdx.x; i < nb; i += TB) m = fmaxf(m, partial_max[i]);
+  m = wave_max(m);
+  __shared__ float sh[TB / 64];
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+  __syncthreads();
+  m = sh[0];
+  for (int i = 1; i < TB / 64; ++i) m = fmaxf(m, sh[i]);
+  if (blockIdx.x == 0 && threadIdx.x == 0 && xmax_out) *xmax_out = m;
+  for (long long e = (long long)blockIdx.x * TB + threadIdx.x; e < n; e += (long long)gridDim.x * TB)
+    x[e] = x[e] > m ? m : x[e];
+}
+
+}  // namespace
+}  // namespace vmbd
+
+using namespace vmbd;
+
+namespace {
+inline int grid_for(long long n) {
+  long long b = (n + TB - 1) / TB;
+  if (b > VMBD_MAX_PARTIALS) b = VMBD_MAX_PARTIALS;
+  return b < 1 ? 1 : (int)b;
+}
+#define SOLVER_DONE(what)                                            \
+  do {                                                               \
+    hipError_t e__ = hipGetLastError();                              \
+    if (e__ != hipSuccess) return vmbd_fail_hip(what, e__);          \
+    return VMBD_OK;                                                  \
+  } while (0)
+}  // namespace
+
+extern "C" {
+
+int vmbd_reduce_blocks(int64_t n) { return grid_for(n); }
+
+int vmbd_clip(float* x, int64_t n, float lb, float ub, int32_t flags, vmbd_stream_t stream) {
+  vmbd_clear_error();
+  if (!x || n < 0) return vmbd_fail(VMBD_E_ARG, "vmbd_clip: bad argument");
+  if (n == 0) return VMBD_OK;
+  hipStream_t st = (hipStream_t)stream;
+  ProfScope ps("clip", st);
+  hipLaunchKernelGGL(k_clip, dim3(grid_for(n)), dim3(TB), 0, st, x, (long long)n, lb, ub, flags);
+  SOLVER_DONE("vmbd_clip");
+}
+
+int vmbd_psf_normalize(float* f, int32_t K, int32_t len, float eps, vmbd_stream_t stream) {
+  vmbd_clear_error();
+  if (!f || K <= 0 || len <= 0) return vmbd_fail(VMBD_E_ARG, "vmbd_psf_normalize: bad argument");
+  hipStream_t st = (hipStream_t)stream;
+  ProfScope ps("psf_normalize", st);
+  hipLaunchKernelGGL(k_normalize, dim3(K), dim3(TB), 0, st, f, len, eps);
+  SOLVER_DONE("vmbd_psf_normalize");
+}
+
+int vmbd_dot(const float* a, const float* b, int64_t n, double* partials, double* out, vmbd_stream_t stream) {
+  vmbd_clear_error();
+  if (!a || !b || !partials || !out || n < 0) return vmbd_fail(VMBD_E_ARG, "vmbd_dot: bad argument");
+  hipStream_t st = (hipStream_t)stream;
+  const int nb = grid_for(n);
+  ProfScope ps("dot", st);
+  hipLaunchKernelGGL(k_dot_partials, dim3(nb), dim3(TB), 0, st, a, b, (long long)n, partials);
+  hipLaunchKernelGGL(k_reduce_out, dim3(1), dim3(TB), 0, st, partials, nb, 1, out);
+  SOLVER_DONE("vmbd_dot");
+}
+
+int vmbd_pbb_stats(const float* x, const float* oldx, const float* g, const float* oldg, int64_t n,
+                   int32_t unconstrained, double* partials, double* sums, vmbd_stream_t stream) {
+  vmbd_clear_error();
+  if (!x || !oldx || !g || !oldg || !partials || n <= 0) return vmbd_fail(VMBD_E_ARG, "vmbd_pbb_stats: bad argument");
+  hipStream_t st = (hipStream_t)stream;
+  const int nb = grid_for(n);
+  ProfScope ps("pbb_stats", st);
+  hipLaunchKernelGGL(k_pbb_stats, dim3(nb), dim3(TB), 0, st, x, oldx, g, oldg, (long long)n, unconstrained, partials);
+  if (sums) hipLaunchKernelGGL(k_reduce_out, dim3(1), dim3(TB), 0, st, partials, nb, 3, sums);
+  SOLVER_DONE("vmbd_pbb_stats");
+}
+
+int vmbd_pbb_update(float* xnew, const float* x, const float* g, int64_t n, const double* partials, int32_t iter,
+                    int32_t unconstrained, float* step_out, vmbd_stream_t stream) {
+  vmbd_clear_error();
+  if (!xnew || !x || !g || !partials || n <= 0) return vmbd_fail(VMBD_E_ARG, "vmbd_pbb_update: bad argument");
+  hipStream_t st = (hipStream_t)stream;
+  const int nb = grid_for(n);
+  ProfScope ps("pbb_update", st);
+  hipLaunchKernelGGL(k_pbb_update, dim3(nb), dim3(TB), 0, st, xnew, x, g, (long long)n, partials, nb, iter,
+                     unconstrained, step_out);
+  SOLVER_DONE("vmbd_pbb_update");
+}
+
+int vmbd_axpby(float* out, const float* a, float s, const float* b, float t, int64_t n, vmbd_stream_t stream) {
+  vmbd_clear_error();
+  if (!out || !a || n < 0) return vmbd_fail(VMBD_E_ARG, "vmbd_axpby: bad argument");
+  if (n == 0) return VMBD_OK;
+  hipStream_t st = (hipStream_t)stream;
+  ProfScope ps("axpby", st);
+  hipLaunchKernelGGL(k_axpby, dim3(grid_for(n)), dim3(TB), 0, st, out, a, s, b, t, (long long)n);
+  SOLVER_DONE("vmbd_axpby");
+}
+
+int vmbd_mul(float* out, const float* a, const float* b, int64_t n, vmbd_stream_t stream) {
+  vmbd_clear_error();
+  if (!out || !a || !b || n < 0) return vmbd_fail(VMBD_E_ARG, "vmbd_mul: bad argument");
+  if (n == 0) return VMBD_OK;
+  hipStream_t st = (hipStream_t)stream;
+  ProfScope ps("mul", st);
+  hipLaunchKernelGGL(k_mul, dim3(grid_for(n)), dim3(TB), 0, st, out, a, b, (long long)n);
+  SOLVER_DONE("vmbd_mul");
+}
+
+int vmbd_ew(float* out, const float* a, const float* b, float s, int64_t n, int32_t op, vmbd_stream_t stream) {
+  vmbd_clear_error();
+  const bool needb = op == VMBD_EW_MUL || op == VMBD_EW_DIV || op == VMBD_EW_SUB || op == VMBD_EW_AXPY;
+  if (!out || !a || (needb && !b) || n < 0 || op < 0 || op > VMBD_EW_SCALE)
+    return vmbd_fail(VMBD_E_ARG, "vmbd_ew: bad argument");
+  if (n == 0) return VMBD_OK;
+  hipStream_t st = (hipStream_t)stream;
+  ProfScope ps("ew", st);
+  hipLaunchKernelGGL(k_ew, dim3(grid_for(n)), dim3(TB), 0, st, out, a, b, s, (long long)n, op);
+  SOLVER_DONE("vmbd_ew");
+}
+
+int vmbd_rl_update(float* x, const float* num, const float* den, int32_t H, int32_t W, float tol, int32_t border,
+                   float* partial_max, float* xmax_out, vmbd_stream_t stream) {
+  vmbd_clear_error();
+  if (!x || !num || !den || !partial_max || H <= 0 || W <= 0 || border < 0 || 2 * border >= H || 2 * border >= W)
+    return vmbd_fail(VMBD_E_ARG, "vmbd_rl_update: bad argument");
+  hipStream_t st = (hipStream_t)stream;
+  const long long n = (long long)H * W;
+  const int nb = grid_for(n);
+  ProfScope ps("rl_update", st);
+  hipLaunchKernelGGL(k_rl_step, dim3(nb), dim3(TB), 0, st, x, num, den, H, W, tol, border, partial_max);
+  hipLaunchKernelGGL(k_rl_clip, dim3(nb), dim3(TB), 0, st, x, n, partial_max, nb, xmax_out);
+  SOLVER_DONE("vmbd_rl_update");
+}
+
+}  // extern "C"
