@@ -166,8 +166,8 @@ def test_solver_primitives(cuda):
     sums = torch.empty(3, dtype=torch.float64, device=cuda)
     step = torch.empty(1, dtype=torch.float32, device=cuda)
     out = torch.empty_like(X)
-    dx = xv.astype(np.float64) - ox
-    dg = gv.astype(np.float64) - og
+    dx = (xv - ox).astype(np.float64)      # differences in float32, sums in float64
+    dg = (gv - og).astype(np.float64)
     m = (xv == 0) & (gv > 0)
     dx[m] = 0
     dg[m] = 0

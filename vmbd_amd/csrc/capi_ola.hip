@@ -282,8 +282,8 @@ vmbd_plan* vmbd_plan_create(const vmbd_geom* gm, const float* win_table, int32_t
     vmbd_fail(VMBD_E_ARG, "vmbd_plan_create: patch grid covers %lldx%lld < image %dx%d", sxp0, sxp1, G.H, G.W);
     return nullptr;
   }
-  if (G.max_frames < 0 || G.max_frames > 65536) {
-    vmbd_fail(VMBD_E_ARG, "vmbd_plan_create: max_frames=%d outside [0, 65536]", G.max_frames);
+  if (G.max_frames < 0 || G.max_frames > 65535) {
+    vmbd_fail(VMBD_E_ARG, "vmbd_plan_create: max_frames=%d outside [0, 65535]", G.max_frames);
     return nullptr;
   }
   for (int k = 0; k < g.Pf; ++k) {

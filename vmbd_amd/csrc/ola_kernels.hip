@@ -45,20 +45,23 @@ __global__ __launch_bounds__(WG) void k_rows_r2c(Geo g, RowJobs jobs, const floa
   constexpr int SL = FftLds<TW>::floats2;
   __shared__ float2 sm[cmax<TPW * SL, 2 * TPW * NBN>::v];
 
-  // grid: x = row-pair blocks of all jobs (job-major), y = patch (frame-major)
+  // grid: x = patch (frame-major), y = row-pair blocks of all jobs (job-major).
+  // Consecutive workgroups are consecutive patches, so the round-robin
+  // placement of workgroups on the XCDs spreads every row block evenly.
+  const int by = blockIdx.y;
   int ji = 0;
   int blk0 = 0;
-  if (jobs.n > 1 && (int)blockIdx.x >= jobs.blk_end[0]) {
+  if (jobs.n > 1 && by >= jobs.blk_end[0]) {
     ji = 1;
     blk0 = jobs.blk_end[0];
-    if (jobs.n > 2 && (int)blockIdx.x >= jobs.blk_end[1]) {
+    if (jobs.n > 2 && by >= jobs.blk_end[1]) {
       ji = 2;
       blk0 = jobs.blk_end[1];
     }
   }
   const RowSrc s = jobs.s[ji];
   float2* __restrict__ out = jobs.out[ji];
-  const int q0 = ((int)blockIdx.x - blk0) * TPW;  // first row pair of the block
+  const int q0 = (by - blk0) * TPW;  // first row pair of the block
 
   const int tid = threadIdx.x;
   const int tr = tid / TPT;
@@ -67,7 +70,7 @@ __global__ __launch_bounds__(WG) void k_rows_r2c(Geo g, RowJobs jobs, const floa
   const int NQ = (nr + 1) >> 1;
   const int q = q0 + tr;
   const bool active = q < NQ;
-  const int p = blockIdx.y;
+  const int p = blockIdx.x;
   const int r0 = s.rlo + 2 * q;
   const int fr = p / g.Pf;
   const int k = p - fr * g.Pf;
@@ -162,7 +165,7 @@ static hipError_t rows_r2c_t(const Geo& g, const RowSrc* s, float2* const* out, 
   }
   if (blocks <= 0 || g.P <= 0) return hipSuccess;
   ProfScope ps(n == 1 ? "rows_r2c" : "rows_r2c_multi", st);
-  hipLaunchKernelGGL(k_rows_r2c<TW>, dim3((unsigned)blocks, (unsigned)g.P), dim3(WG), 0, st, g, jobs, tw);
+  hipLaunchKernelGGL(k_rows_r2c<TW>, dim3((unsigned)g.P, (unsigned)blocks), dim3(WG), 0, st, g, jobs, tw);
   return hipGetLastError();
 }
 
@@ -235,8 +238,8 @@ __global__ __launch_bounds__(WG) void k_cols(Geo g, ColIn a, ColIn b, ColIn c3, 
   __shared__ float2 sm[(OP == COL_SPEC) ? CS * SL : cmax<CS * SL, TH * (CS + 1)>::v];
 
   const int NB = g.NB;
-  const int p = blockIdx.y;
-  const int strip = blockIdx.x;
+  const int p = blockIdx.x;
+  const int strip = blockIdx.y;
   const int cc = threadIdx.x / TPT;
   const int t = threadIdx.x - cc * TPT;
   const int c = strip * CS + cc;
@@ -280,7 +283,7 @@ static hipError_t cols_t(const Geo& g, int op, const ColIn& a, const ColIn& b, c
                          const ColOut& o2, float scale, float2* spec_out, const float2* tw, hipStream_t st) {
   constexpr int CS = WG / (TH / 16);
   const int strips = (g.NB + CS - 1) / CS;
-  const dim3 grid((unsigned)strips, (unsigned)g.P);
+  const dim3 grid((unsigned)g.P, (unsigned)strips);
   static const char* names[4] = {"cols_spec", "cols_fwd", "cols_adj1", "cols_adj2"};
   ProfScope ps(names[op & 3], st);
   switch (op) {
@@ -497,9 +500,9 @@ __global__ __launch_bounds__(WG) void k_rows_c2r_stack(Geo g, const float2* __re
   const int tr = threadIdx.x / TPT;
   const int t = threadIdx.x - tr * TPT;
   const int NQ = (nrows + 1) >> 1;
-  const int q = blockIdx.x * TPW + tr;
+  const int q = blockIdx.y * TPW + tr;
   const bool act = q < NQ;
-  const int p = blockIdx.y;
+  const int p = blockIdx.x;
   const int r0 = 2 * q, r1 = 2 * q + 1;
   const float2* ra = act ? src + ((size_t)p * srow + r0) * g.NB : src;
   const bool hasb = act && r1 < nrows;
@@ -529,7 +532,7 @@ static hipError_t c2r_stack_t(const Geo& g, const float2* src, int srow, float* 
   const int blocks = ((nrows + 1) / 2 + TPW - 1) / TPW;
   if (blocks <= 0 || g.P <= 0) return hipSuccess;
   ProfScope ps("rows_c2r_stack", st);
-  hipLaunchKernelGGL(k_rows_c2r_stack<TW>, dim3((unsigned)blocks, (unsigned)g.P), dim3(WG), 0, st, g, src, srow,
+  hipLaunchKernelGGL(k_rows_c2r_stack<TW>, dim3((unsigned)g.P, (unsigned)blocks), dim3(WG), 0, st, g, src, srow,
                      out, nrows, ncols, tw);
   return hipGetLastError();
 }

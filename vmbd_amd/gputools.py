@@ -33,6 +33,15 @@ def _p(t):
     return _lib.ptr(t)
 
 
+def _inplace(t, name):
+    """Check that an in-place operand is a contiguous float32 device tensor
+    (the kernels address it as one flat array of numel() elements)."""
+    if not (isinstance(t, torch.Tensor) and t.device.type == 'cuda' and t.dtype == torch.float32
+            and t.is_contiguous()):
+        raise TypeError(f'{name}: expected a contiguous float32 device tensor')
+    return t
+
+
 def _st():
     return _lib.stream_ptr()
 
@@ -149,6 +158,8 @@ def ola_GPU_test(xs_gpu, csf, sw, nhop, offset=(0, 0)):
     w0, w1 = _pair(sw)
     h0, h1 = _pair(nhop)
     o0, o1 = _pair(offset)
+    if xs.dim() != 3 or xs.shape[0] < c0 * c1:
+        raise IOError('ola_GPU_test: the stack holds fewer than prod(csf) slices')
     out = torch.empty((h0 * c0 + w0 - h0, h1 * c1 + w1 - h1), dtype=torch.float32, device=xs.device)
     _lib.call('vmbd_ola', _p(out), _p(xs), xs.shape[1], xs.shape[2], c0, c1, w0, w1, h0, h1, o0, o1, int(cplx), _st())
     return out
@@ -187,6 +198,8 @@ def dot(a, b):
     a = _dev(a)
     b = _dev(b)
     n = a.numel()
+    if b.numel() != n:
+        raise ValueError('dot: operand sizes differ')
     part = torch.empty(3 * _lib.lib().vmbd_reduce_blocks(n), dtype=torch.float64, device=a.device)
     out = torch.empty(1, dtype=torch.float64, device=a.device)
     _lib.call('vmbd_dot', _p(a), _p(b), n, _p(part), _p(out), _st())
@@ -196,13 +209,26 @@ def dot(a, b):
 # ------------------------------------------------------ Fourier divisions
 def _cdev(t):
     t = t if isinstance(t, torch.Tensor) else _dev(t, _C)
+    if t.device.type != 'cuda':
+        t = t.to('cuda')
     return t.to(_C).contiguous()
+
+
+def _check_spectra(stacks, tiles, per):
+    """Per-patch spectra share one element count; broadcast operands hold one
+    tile of `per` elements."""
+    n = stacks[0].numel()
+    if n % per or any(s.numel() != n for s in stacks):
+        raise ValueError('spectrum stacks differ in size')
+    if any(t.numel() != per for t in tiles):
+        raise ValueError('broadcast spectra must hold exactly one tile')
 
 
 def comp_ola_deconv(fs_gpu, ys_gpu, L_gpu, alpha, beta):
     """gputools.py:498-519 (gputools.cu:316-341)."""
     f, y, L = _cdev(fs_gpu), _cdev(ys_gpu), _cdev(L_gpu)
     per = f.shape[-2] * f.shape[-1]
+    _check_spectra((f, y), (L,), per)
     out = torch.empty_like(f)
     _lib.call('vmbd_fourier_div', _p(out), _p(f), _p(y), _p(L), f.numel(), per, float(alpha), float(beta), _st())
     return out
@@ -212,6 +238,7 @@ def comp_ola_gdeconv(xx_gpu, xy_gpu, yx_gpu, yy_gpu, L_gpu, alpha, beta):
     """gputools.py:522-543 (gputools.cu:343-374)."""
     xx, xy, yx, yy, L = (_cdev(v) for v in (xx_gpu, xy_gpu, yx_gpu, yy_gpu, L_gpu))
     per = xx.shape[-2] * xx.shape[-1]
+    _check_spectra((xx, xy, yx, yy), (L,), per)
     out = torch.empty_like(xx)
     _lib.call('vmbd_fourier_div_g', _p(out), _p(xx), _p(xy), _p(yx), _p(yy), _p(L), xx.numel(), per, float(alpha),
               float(beta), _st())
@@ -222,6 +249,7 @@ def comp_ola_sdeconv(gx_gpu, gy_gpu, xx_gpu, xy_gpu, Ftpy_gpu, f_gpu, L_gpu, alp
     """gputools.py:572-596 (gputools.cu:376-406)."""
     gx, gy, xx, xy, fy, f, L = (_cdev(v) for v in (gx_gpu, gy_gpu, xx_gpu, xy_gpu, Ftpy_gpu, f_gpu, L_gpu))
     per = xx.shape[-2] * xx.shape[-1]
+    _check_spectra((xx, xy, fy, f), (gx, gy, L), per)
     out = torch.empty_like(xx)
     _lib.call('vmbd_fourier_div_s', _p(out), _p(gx), _p(gy), _p(xx), _p(xy), _p(fy), _p(f), _p(L), xx.numel(), per,
               float(alpha), float(beta), float(gamma), _st())
@@ -231,16 +259,19 @@ def comp_ola_sdeconv(gx_gpu, gy_gpu, xx_gpu, xy_gpu, Ftpy_gpu, f_gpu, L_gpu, alp
 # ------------------------------------------------------ clipping, normalise
 def cliplower_GPU(x_gpu, lb):
     """gputools.py:478-485, in place."""
+    _inplace(x_gpu, 'cliplower_GPU')
     _lib.call('vmbd_clip', _p(x_gpu), x_gpu.numel(), float(lb), 0.0, 1, _st())
 
 
 def clipupper_GPU(x_gpu, ub):
     """gputools.py:488-495, in place."""
+    _inplace(x_gpu, 'clipupper_GPU')
     _lib.call('vmbd_clip', _p(x_gpu), x_gpu.numel(), 0.0, float(ub), 2, _st())
 
 
 def clip_GPU(x_gpu, lb, ub):
     """gputools.py:472-475, in place."""
+    _inplace(x_gpu, 'clip_GPU')
     _lib.call('vmbd_clip', _p(x_gpu), x_gpu.numel(), float(lb), float(ub), 3, _st())
 
 
@@ -293,6 +324,7 @@ def edgetaper_windows(shape, sf, win='barthann'):
 
 def edgetaper_gpu(y_gpu, sf, win='barthann'):
     """gputools.py:643-673: in-place border taper of width floor(sf/2)."""
+    _inplace(y_gpu, 'edgetaper_gpu')
     wx, wy = edgetaper_windows(y_gpu.shape, sf, win)
     dwx = torch.as_tensor(wx, device=y_gpu.device) if wx.size else None
     dwy = torch.as_tensor(wy, device=y_gpu.device) if wy.size else None
@@ -352,17 +384,20 @@ def laplace3d_gpu(y_gpu):
 
 def modify_sparse23_gpu(y_gpu, beta):
     """gputools.py:984-1001, in place."""
+    _inplace(y_gpu, 'modify_sparse23_gpu')
     _lib.call('vmbd_shrink_alpha23', _p(y_gpu), y_gpu.numel(), float(beta), _st())
 
 
 def modify_sparse_gpu(y_gpu, beta, alpha=2. / 3.):
     """gputools.py:1004-1022, in place."""
+    _inplace(y_gpu, 'modify_sparse_gpu')
     _lib.call('vmbd_shrink_alpha', _p(y_gpu), y_gpu.numel(), float(beta), float(alpha), _st())
 
 
 def shock_filter_gpu(y_gpu, iter=3, dt=0.1, h=1):
     """gputools.py:1282-1314: `iter` shock steps, in place (each step reads
     the previous image and writes a new one)."""
+    _inplace(y_gpu, 'shock_filter_gpu')
     H, W = y_gpu.shape
     a = y_gpu
     b = torch.empty_like(y_gpu)
@@ -375,6 +410,7 @@ def shock_filter_gpu(y_gpu, iter=3, dt=0.1, h=1):
 
 def bfilter_gpu(y_gpu, w=5, sigma_d=3, sigma_r=0.1):
     """gputools.py:251-285: bilateral filter, in place."""
+    _inplace(y_gpu, 'bfilter_gpu')
     H, W = y_gpu.shape
     out = torch.empty_like(y_gpu)
     _lib.call('vmbd_bilateral', _p(out), _p(y_gpu), H, W, int(w), float(sigma_d), float(sigma_r), _st())

@@ -236,7 +236,7 @@ class OlaGPU:
             gy = self._filter_spectrum([[-1, -1, -1], [1, 1, 1]])
             L = _spectral_energy(gx, gy)
         else:
-            L = torch.zeros(full.shape, dtype=torch.complex64, device=full.device)
+            L = torch.zeros(full.shape[-2:], dtype=torch.complex64, device=full.device)
         if edgetapering == 1:
             gputools.edgetaper_gpu(y_gpu, 2 * self.sf, 'barthann')
         x_gpu = gputools._dev(self.x)
@@ -282,7 +282,7 @@ class OlaGPU:
             gy = self._filter_spectrum([[-1], [1]])
             L = _spectral_energy(gx, gy)
         else:
-            L = torch.zeros(full.shape, dtype=torch.complex64, device=full.device)
+            L = torch.zeros(full.shape[-2:], dtype=torch.complex64, device=full.device)
         if edgetapering == 1:
             gputools.edgetaper_gpu(y_gpu, 3 * self.sf, 'barthann')
         off = self._offset()
