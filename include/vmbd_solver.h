@@ -53,6 +53,13 @@ int vmbd_pbb_stats(const float* x, const float* oldx, const float* g, const floa
 int vmbd_pbb_update(float* xnew, const float* x, const float* g, int64_t n, const double* partials, int32_t iter,
                     int32_t unconstrained, float* step_out, vmbd_stream_t stream);
 
+/* Start pair of PBB (optGPU.py:256-297, replaces the host comparison of
+ * the two start objectives): dots = device [r.r, z.z] at the first point and
+ * at the second, objective r.r/2 + beta z.z/2; when the second point is not
+ * worse, (x, oldx) and (g, oldg) exchange contents.  No host synchronisation. */
+int vmbd_pbb_select(float* x, float* oldx, float* g, float* oldg, int64_t n, const double* dots, double beta,
+                    vmbd_stream_t stream);
+
 /* out = a*s + b*t (b may be NULL). */
 int vmbd_axpby(float* out, const float* a, float s, const float* b, float t, int64_t n, vmbd_stream_t stream);
 /* Elementwise family: out = a*b, a/b, s/a, a-b, a + s*b or s*a (op
@@ -63,17 +70,19 @@ int vmbd_axpby(float* out, const float* a, float s, const float* b, float t, int
 #define VMBD_EW_SUB 3
 #define VMBD_EW_AXPY 4
 #define VMBD_EW_SCALE 5
+#define VMBD_EW_LT 6   /* out = (a < s) ? 1 : 0 (saturation mask, vmbd_chimney.py:117) */
+#define VMBD_EW_ADDS 7 /* out = a + s */
 int vmbd_ew(float* out, const float* a, const float* b, float s, int64_t n, int32_t op, vmbd_stream_t stream);
 
 /* out = a * b elementwise. */
 int vmbd_mul(float* out, const float* a, const float* b, int64_t n, vmbd_stream_t stream);
 
 /* Multiplicative latent-image step (vmbd_chimney.py:143-147):
- * x *= max(num / (den + tol), tol); then x = min(x, max of x over rows and
- * columns [border, size - border)).  partial_max: vmbd_reduce_blocks(H*W)
- * floats; xmax_out (optional) receives the clip level. */
-int vmbd_rl_update(float* x, const float* num, const float* den, int32_t H, int32_t W, float tol, int32_t border,
-                   float* partial_max, float* xmax_out, vmbd_stream_t stream);
+ * x *= max(num / (den + tol), tol); then x = min(x, max of x over rows
+ * [border0, H - border0) and columns [border1, W - border1)).  partial_max:
+ * vmbd_reduce_blocks(H*W) floats; xmax_out (optional) receives the clip level. */
+int vmbd_rl_update(float* x, const float* num, const float* den, int32_t H, int32_t W, float tol, int32_t border0,
+                   int32_t border1, float* partial_max, float* xmax_out, vmbd_stream_t stream);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,132 @@
+"""GPU parity of the online MFBD loop (vmbd_chimney.py:110-147) and of the
+device PBB solver against the FP64 oracles.
+
+Iterated solvers amplify rounding: with every operator output rounded to
+float32, the FP64 oracle's PBB iterates move by about 1e-5 (10-20
+iterations) to 1e-4 (40 iterations) in relative L2, so these comparisons use
+1e-3 after at most 10 iterations."""
+import numpy as np
+import pytest
+import scipy.optimize as so
+
+from oracle import mfbd_oracle as M
+from oracle import ola_oracle as O
+from tests.helpers import case_inputs, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+TOL_ITER = 1e-3
+
+
+def _t(a, cuda):
+    import torch
+    return torch.as_tensor(np.ascontiguousarray(a, dtype=np.float32), device=cuda)
+
+
+def _n(t):
+    return t.detach().cpu().numpy().astype(np.float64)
+
+
+def _scene(seed, sy, sf, csf):
+    """Latent x (sy + sf - 1), PSFs, observation y with a saturated strip, and
+    a blurred first estimate y0."""
+    sx = (sy[0] + sf[0] - 1, sy[1] + sf[1] - 1)
+    x, f, _ = case_inputs(seed, sx, csf, sf)
+    wa = O.Win2WinAux(sx, csf, 0.5)
+    y = O.forward(x, f, wa) * 1.6 + 0.01 * np.random.default_rng(seed + 1).standard_normal(sy)
+    y[:3] = np.maximum(y[:3], 1.2)                  # saturated rows: masked out
+    y0 = O.forward(x, np.full_like(f, 1.0 / (sf[0] * sf[1])), wa)
+    return sx, wa, y.astype(np.float32).astype(np.float64), y0.astype(np.float32).astype(np.float64)
+
+
+@pytest.mark.parametrize('exact', [False, True])
+def test_frame_pbb_rl_vs_oracle(cuda, exact):
+    from vmbd_amd.mfbd import OnlineMFBD
+    sy, sf, csf = (64, 64), (7, 7), (3, 3)
+    print(f'case mfbd pbb+rl exact={exact}', flush=True)
+    sx, wa, y, y0 = _scene(11, sy, sf, csf)
+    m = OnlineMFBD(y0, sf, csf, f_solver='pbb', x_solver='rl', f_iters=10, f_beta=0.1, exact=exact)
+    x0 = _n(m.x)
+    np.testing.assert_array_equal(x0, np.pad(y0.astype(np.float32), ((3, 3), (3, 3)), mode='edge'))
+    x1, f1 = m.step(y)
+    fu = _n(m.fu)
+    xr, fr = M.frame_step(x0, y, wa, sf, fu, 10, 0.1, tol=1e-10, exact=exact)
+    assert rel_l2(_n(f1), fr) < TOL_ITER
+    assert rel_l2(_n(x1), xr) < TOL_ITER
+    # the x-step alone, from the PSFs the device produced: one pass, operator tolerance
+    xr2 = M.rl_step(x0, _n(f1), y, wa, sf, tol=1e-10, exact=exact)
+    assert rel_l2(_n(x1), xr2) < 1e-5
+
+
+def test_frame_pbb_pbb_vs_oracle(cuda):
+    from vmbd_amd.mfbd import OnlineMFBD
+    sy, sf, csf = (60, 72), (5, 5), (2, 3)
+    print('case mfbd pbb+pbb', flush=True)
+    sx, wa, y, y0 = _scene(12, sy, sf, csf)
+    m = OnlineMFBD(y0, sf, csf, f_solver='pbb', x_solver='pbb', f_iters=8, x_iters=6, f_beta=0.05, exact=True)
+    x0 = _n(m.x)
+    x1, f1 = m.step(y)
+    xr, fr = M.frame_step(x0, y, wa, sf, _n(m.fu), 8, 0.05, exact=True, x_solver='pbb', xu=_n(m.xu), x_iters=6)
+    assert rel_l2(_n(f1), fr) < TOL_ITER
+    assert rel_l2(_n(x1), xr) < TOL_ITER
+
+
+def test_capture_replays_match_eager(cuda):
+    """Three frames replayed from one recorded graph equal three eager frames
+    bit for bit (deterministic reductions, same random draws)."""
+    import torch
+    from vmbd_amd.mfbd import OnlineMFBD
+    sy, sf, csf = (64, 64), (7, 7), (3, 3)
+    print('case mfbd capture', flush=True)
+    _, _, y, y0 = _scene(13, sy, sf, csf)
+    kw = dict(f_solver='pbb', x_solver='rl', f_iters=6, f_beta=0.1, seed=7)
+    a = OnlineMFBD(y0, sf, csf, capture=False, **kw)
+    b = OnlineMFBD(y0, sf, csf, capture=True, **kw)
+    rng = np.random.default_rng(5)
+    for i in range(3):
+        yi = y + 0.01 * rng.standard_normal(y.shape)
+        xa, fa = a.step(yi)
+        xa, fa = xa.clone(), fa.clone()
+        torch.cuda.synchronize()
+        xb, fb = b.step(yi)
+        torch.cuda.synchronize()
+        assert torch.equal(xa, xb), i
+        assert torch.equal(fa, fb), i
+
+
+def test_frame_lbfgsb(cuda):
+    """The reference's f-solver (host L-BFGS-B over device operators): the
+    PSFs come back normalised and nonnegative, and the x-step from them
+    matches the oracle's multiplicative step."""
+    from vmbd_amd.mfbd import OnlineMFBD
+    sy, sf, csf = (64, 64), (7, 7), (3, 3)
+    print('case mfbd lbfgsb', flush=True)
+    sx, wa, y, y0 = _scene(14, sy, sf, csf)
+    m = OnlineMFBD(y0, sf, csf, f_solver='lbfgsb', x_solver='rl', optiter=30, f_beta=0.1)
+    x0 = _n(m.x)
+    x1, f1 = m.step(y)
+    f1 = _n(f1)
+    assert np.all(f1 >= 0)
+    np.testing.assert_allclose(f1.sum(axis=(1, 2)), 1.0, rtol=1e-5)
+    xr = M.rl_step(x0, f1, y, wa, sf, tol=1e-10)
+    assert rel_l2(_n(x1), xr) < 1e-5
+
+
+def test_pbb_class_converges_to_nnls(cuda):
+    """optGPU.PBB on the dense MVM objective (optGPU.py:19-31) reaches the
+    NNLS optimum (scaled so the 1e-5 BB guard sits below float32 accuracy)."""
+    import torch
+    from vmbd_amd import optGPU
+    print('case PBB class', flush=True)
+    rng = np.random.default_rng(1)
+    A = 100 * rng.standard_normal((40, 12))
+    b = 100 * rng.standard_normal(40)
+    ref, _ = so.nnls(A, b)
+    opt = optGPU.Solopt()
+    opt.maxiter = 300
+    opt.tolo = 0.0
+    obj = optGPU.MVM_Objective(_t(A, cuda), _t(b, cuda))
+    res = optGPU.PBB(obj, _t(rng.random(12) + 0.1, cuda), opt)
+    assert res.iter == 300
+    assert rel_l2(_n(res.result), ref) < 1e-4
+    assert torch.all(res.result >= 0)

@@ -164,6 +164,24 @@ __global__ __launch_bounds__(TB) void k_pbb_update(float* xnew, const float* x, 
   }
 }
 
+// PBB start (optGPU.py:256-297): the iterate is the start point with the
+// lower objective.  dots = [r.r, z.z] at the first point, then at the second;
+// objective = r.r/2 + beta z.z/2.  When the second point is not worse the
+// pairs (x, oldx) and (g, oldg) exchange contents.
+__global__ __launch_bounds__(TB) void k_pbb_select(float* x, float* oldx, float* g, float* oldg, long long n,
+                                                   const double* dots, double beta) {
+  const double fx = 0.5 * dots[0] + 0.5 * beta * dots[1];
+  const double fy = 0.5 * dots[2] + 0.5 * beta * dots[3];
+  if (fx < fy) return;
+  for (long long e = (long long)blockIdx.x * TB + threadIdx.x; e < n; e += (long long)gridDim.x * TB) {
+    const float a = x[e], b = g[e];
+    x[e] = oldx[e];
+    oldx[e] = a;
+    g[e] = oldg[e];
+    oldg[e] = b;
+  }
+}
+
 // out = a*s + b*t (residuals, Tikhonov terms)
 __global__ __launch_bounds__(TB) void k_axpby(float* out, const float* a, float s, const float* b, float t,
                                               long long n) {
@@ -182,6 +200,8 @@ __global__ __launch_bounds__(TB) void k_ew(float* out, const float* a, const flo
       case VMBD_EW_SDIV: r = s / x; break;
       case VMBD_EW_SUB: r = x - b[e]; break;
       case VMBD_EW_AXPY: r = x + s * b[e]; break;
+      case VMBD_EW_LT: r = x < s ? 1.f : 0.f; break;
+      case VMBD_EW_ADDS: r = x + s; break;
       default: r = s * x; break;
     }
     out[e] = r;
@@ -196,9 +216,9 @@ __global__ __launch_bounds__(TB) void k_mul(float* out, const float* a, const fl
 
 // Multiplicative x-step of the online loop (vmbd_chimney.py:143-147), stage 1:
 // x *= max(num/(den + tol), tol); block maxima of x over the interior
-// rows/cols [border, size-border).
+// rows [b0, H-b0) and columns [b1, W-b1).
 __global__ __launch_bounds__(TB) void k_rl_step(float* x, const float* num, const float* den, int H, int W, float tol,
-                                                int border, float* partial_max) {
+                                                int b0, int b1, float* partial_max) {
   float mx = -INFINITY;
   const long long n = (long long)H * W;
   for (long long e = (long long)blockIdx.x * TB + threadIdx.x; e < n; e += (long long)gridDim.x * TB) {
@@ -207,7 +227,7 @@ __global__ __launch_bounds__(TB) void k_rl_step(float* x, const float* num, cons
     const float v = x[e] * fac;
     x[e] = v;
     const int i = (int)(e / W), j = (int)(e - (long long)(e / W) * W);
-    if (i >= border && i < H - border && j >= border && j < W - border) mx = fmaxf(mx, v);
+    if (i >= b0 && i < H - b0 && j >= b1 && j < W - b1) mx = fmaxf(mx, v);
   }
   mx = wave_max(mx);
   __shared__ float sh[TB / 64];
@@ -313,6 +333,16 @@ int vmbd_pbb_update(float* xnew, const float* x, const float* g, int64_t n, cons
   SOLVER_DONE("vmbd_pbb_update");
 }
 
+int vmbd_pbb_select(float* x, float* oldx, float* g, float* oldg, int64_t n, const double* dots, double beta,
+                    vmbd_stream_t stream) {
+  vmbd_clear_error();
+  if (!x || !oldx || !g || !oldg || !dots || n <= 0) return vmbd_fail(VMBD_E_ARG, "vmbd_pbb_select: bad argument");
+  hipStream_t st = (hipStream_t)stream;
+  ProfScope ps("pbb_select", st);
+  hipLaunchKernelGGL(k_pbb_select, dim3(grid_for(n)), dim3(TB), 0, st, x, oldx, g, oldg, (long long)n, dots, beta);
+  SOLVER_DONE("vmbd_pbb_select");
+}
+
 int vmbd_axpby(float* out, const float* a, float s, const float* b, float t, int64_t n, vmbd_stream_t stream) {
   vmbd_clear_error();
   if (!out || !a || n < 0) return vmbd_fail(VMBD_E_ARG, "vmbd_axpby: bad argument");
@@ -336,7 +366,7 @@ int vmbd_mul(float* out, const float* a, const float* b, int64_t n, vmbd_stream_
 int vmbd_ew(float* out, const float* a, const float* b, float s, int64_t n, int32_t op, vmbd_stream_t stream) {
   vmbd_clear_error();
   const bool needb = op == VMBD_EW_MUL || op == VMBD_EW_DIV || op == VMBD_EW_SUB || op == VMBD_EW_AXPY;
-  if (!out || !a || (needb && !b) || n < 0 || op < 0 || op > VMBD_EW_SCALE)
+  if (!out || !a || (needb && !b) || n < 0 || op < 0 || op > VMBD_EW_ADDS)
     return vmbd_fail(VMBD_E_ARG, "vmbd_ew: bad argument");
   if (n == 0) return VMBD_OK;
   hipStream_t st = (hipStream_t)stream;
@@ -345,16 +375,18 @@ int vmbd_ew(float* out, const float* a, const float* b, float s, int64_t n, int3
   SOLVER_DONE("vmbd_ew");
 }
 
-int vmbd_rl_update(float* x, const float* num, const float* den, int32_t H, int32_t W, float tol, int32_t border,
-                   float* partial_max, float* xmax_out, vmbd_stream_t stream) {
+int vmbd_rl_update(float* x, const float* num, const float* den, int32_t H, int32_t W, float tol, int32_t border0,
+                   int32_t border1, float* partial_max, float* xmax_out, vmbd_stream_t stream) {
   vmbd_clear_error();
-  if (!x || !num || !den || !partial_max || H <= 0 || W <= 0 || border < 0 || 2 * border >= H || 2 * border >= W)
+  if (!x || !num || !den || !partial_max || H <= 0 || W <= 0 || border0 < 0 || border1 < 0 || 2 * border0 >= H ||
+      2 * border1 >= W)
     return vmbd_fail(VMBD_E_ARG, "vmbd_rl_update: bad argument");
   hipStream_t st = (hipStream_t)stream;
   const long long n = (long long)H * W;
   const int nb = grid_for(n);
   ProfScope ps("rl_update", st);
-  hipLaunchKernelGGL(k_rl_step, dim3(nb), dim3(TB), 0, st, x, num, den, H, W, tol, border, partial_max);
+  hipLaunchKernelGGL(k_rl_step, dim3(nb), dim3(TB), 0, st, x, num, den, H, W, tol, border0, border1,
+                     partial_max);
   hipLaunchKernelGGL(k_rl_clip, dim3(nb), dim3(TB), 0, st, x, n, partial_max, nb, xmax_out);
   SOLVER_DONE("vmbd_rl_update");
 }

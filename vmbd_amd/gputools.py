@@ -166,12 +166,12 @@ def ola_GPU_test(xs_gpu, csf, sw, nhop, offset=(0, 0)):
 
 
 # ------------------------------------------------------ elementwise helpers
-EW_MUL, EW_DIV, EW_SDIV, EW_SUB, EW_AXPY, EW_SCALE = range(6)
+EW_MUL, EW_DIV, EW_SDIV, EW_SUB, EW_AXPY, EW_SCALE, EW_LT, EW_ADDS = range(8)
 
 
 def ew(op, a, b=None, s=0.0, out=None):
     """Elementwise float32 op on the device (vmbd_ew): a*b, a/b, s/a, a-b,
-    a+s*b, s*a."""
+    a+s*b, s*a, (a<s), a+s."""
     a = _dev(a)
     b = _dev(b) if b is not None else None
     if b is not None and b.shape != a.shape:

@@ -126,8 +126,9 @@ class OlaPlan:
     def _empty(self, shape, out):
         if out is None:
             return torch.empty(shape, dtype=torch.float32, device=self.device)
-        if tuple(out.shape) != tuple(shape) or out.dtype != torch.float32 or not out.is_contiguous():
-            raise ValueError('out has the wrong shape/dtype/layout')
+        if (tuple(out.shape) != tuple(shape) or out.dtype != torch.float32 or not out.is_contiguous()
+                or out.device != self.device):
+            raise ValueError('out has the wrong shape/dtype/layout/device')
         return out
 
     @staticmethod
@@ -196,9 +197,9 @@ class OlaPlan:
         return xb, fb
 
     # -- cached spectra (OlaGPU semantics) ----------------------------------
-    def spectra(self, kind, z):
+    def spectra(self, kind, z, out=None):
         """kind 'F': z = PSF stack; kind 'X': z = image.  Returns a complex64
-        tensor of shape ([B,] P, Tw/2+1, Th)."""
+        tensor of shape ([B,] P, Tw/2+1, Th) (written into `out` if given)."""
         if kind == 'F':
             z, nf, bat = self._f(z)
             k = _lib.CACHE_PSF
@@ -207,8 +208,14 @@ class OlaPlan:
             k = _lib.CACHE_IMAGE
         else:
             raise ValueError("kind must be 'F' or 'X'")
-        spec = torch.empty(self._lead(nf, bat) + (self.P, self.tile[1] // 2 + 1, self.tile[0]),
-                           dtype=torch.complex64, device=self.device)
+        shape = self._lead(nf, bat) + (self.P, self.tile[1] // 2 + 1, self.tile[0])
+        if out is None:
+            spec = torch.empty(shape, dtype=torch.complex64, device=self.device)
+        else:
+            if (tuple(out.shape) != shape or out.dtype != torch.complex64 or not out.is_contiguous()
+                    or out.device != self.device):
+                raise ValueError(f'out must be a contiguous complex64 tensor of shape {shape}')
+            spec = out
         self._rows_key = None
         _lib.check(_lib.lib().vmbd_spectra(self._h, nf, k, _lib.ptr(z), _lib.ptr(spec), _lib.stream_ptr()),
                    'spectra')
