@@ -11,8 +11,13 @@ images, random-walk PSFs, Gaussian residuals) and already resident in HBM
 when the timed region starts.
 
 Prints ONE JSON line.  value = Mpix/s = frames * 1024^2 / time per step.
+Beside it: the same step in the plain per-operator GPU form (the
+reference's launch sequence, one kernel per operation, full complex spectra)
+and the online MFBD loop in frames/s (1024^2 latent, 20 projected-BB
+f-iterations + 20 x-iterations per frame, graph-captured).
 
     python bench.py [--steps K] [--warmup W] [--frames B] [--no-graph] [--cpu-seconds S]
+                    [--no-plain] [--no-mfbd] [--mfbd-frames N]
 """
 import argparse
 import json
@@ -39,6 +44,9 @@ def parse():
     ap.add_argument('--no-graph', action='store_true', help='launch eagerly instead of replaying a HIP graph')
     ap.add_argument('--cpu-seconds', type=float, default=12.0, help='bounded CPU-baseline sample')
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--no-plain', action='store_true', help='skip the plain per-operator comparator')
+    ap.add_argument('--no-mfbd', action='store_true', help='skip the MFBD frames/s measurement')
+    ap.add_argument('--mfbd-frames', type=int, default=10, help='timed MFBD frames')
     ap.add_argument('--calib', action='store_true',
                     help='run one 64 MiB elementwise copy first (counter calibration)')
     return ap.parse_args()
@@ -82,6 +90,58 @@ def operator_bytes():
     Sy = (H - SF[0] + 1) * (W - SF[1] + 1)
     Pf = CSF[0] * CSF[1] * SF[0] * SF[1]
     return 4.0 * 3 * (Sx + Sy + Pf)
+
+
+def plain_form(wa, x, f, g, timed, a):
+    """fwd + both transposes of one frame in the plain per-operator form."""
+    from vmbd_amd.plain import PlainOla
+    p = PlainOla((H, W), SF, wa)
+    steps = max(3, min(a.steps, 10))
+    for _ in range(2):
+        p.step(x, f, g)
+    ms = timed(lambda: p.step(x, f, g), steps)
+    return {'ms_per_frame': round(ms, 4), 'mpix_s': round(H * W / (ms * 1e-3) / 1e6, 2), 'steps': steps,
+            'form': 'reference launch sequence (F ctor, X ctor, F.cnv, F.cnvtp, X.cnvtp), one HIP kernel per '
+                    'operation, complex (T x T) spectra, patch-local transposes'}
+
+
+def mfbd_rate(sx, csf, sf, walk, nframes, timed):
+    """Online MFBD frames/s: latent sx, PSF grid csf of sf PSFs, per frame
+    20 PBB f-iterations + 20 PBB x-iterations (exact adjoints), captured in
+    a HIP graph and replayed per frame."""
+    import numpy as np
+    import torch
+
+    from vmbd_amd import synth
+    from vmbd_amd.imagetools import win2winaux
+    from vmbd_amd.mfbd import OnlineMFBD
+    from vmbd_amd.ola import OlaPlan
+
+    dev = torch.device('cuda', 0)
+    rng = np.random.default_rng(99)
+    wa = win2winaux(sx, csf, OVERLAP)
+    plan = OlaPlan(sx, sf, wa, 'valid', device=dev)
+    xt = torch.as_tensor(synth.piecewise_smooth(sx, rng).astype(np.float32), device=dev)
+    obs = []
+    for _ in range(4):
+        ft = torch.as_tensor(synth.motion_psfs(plan.P, sf, rng, steps=walk).astype(np.float32), device=dev)
+        yt = plan.forward(xt, ft)
+        noise = torch.as_tensor((0.01 * rng.standard_normal(plan.sy)).astype(np.float32), device=dev)
+        obs.append(yt + noise)
+    m = OnlineMFBD(obs[0], sf, csf, f_solver='pbb', x_solver='pbb', f_iters=20, x_iters=20, f_beta=1e-3,
+                   exact=True, capture=True, device=dev)
+    for i in range(2):
+        m.step(obs[i % 4])
+    k = [0]
+
+    def one():
+        m.step(obs[k[0] % 4])
+        k[0] += 1
+
+    ms = timed(one, nframes)
+    return {'frames_per_s': round(1e3 / ms, 2), 'ms_per_frame': round(ms, 3), 'frames': nframes,
+            'latent': list(sx), 'csf': list(csf), 'sf': list(sf), 'f_iters': 20, 'x_iters': 20,
+            'graph': True}
 
 
 def main():
@@ -195,6 +255,17 @@ def main():
     op_bytes = operator_bytes() * B
     step_hbm = op_bytes / (ms * 1e-3) / 1e9
 
+    plain = None
+    if not a.no_plain:
+        plain = plain_form(wa, x[0], f[0], g[0], timed, a)
+        plain['fused_speedup'] = round(mpix / plain['mpix_s'], 2)
+        plain['fused_speedup_single_frame_eager'] = round(H * W / (ms1 * 1e-3) / 1e6 / plain['mpix_s'], 2)
+
+    mfbd = None
+    if not a.no_mfbd:
+        mfbd = {'1024': mfbd_rate((H, W), CSF, SF, 40, a.mfbd_frames, timed),
+                '512': mfbd_rate((512, 512), (8, 8), (31, 31), 15, a.mfbd_frames, timed)}
+
     cpu = None
     acc = None
     if not a.no_cpu:
@@ -229,7 +300,7 @@ def main():
                    fbar=rl2(fb[0].cpu().numpy().astype(np.float64), fb_ref))
 
     out = {
-        'metric': 'OLA fwd+adj Mpix/s at 1024^2 FP32 (rel err <= 1e-5); %HBM peak',
+        'metric': 'OLA fwd+adj Mpix/s & MFBD frames/s at 1024\u00b2 FP32 (rel err \u22641e-5); %HBM peak',
         'value': round(mpix, 2),
         'unit': 'Mpix/s',
         'n_gpus': 1,
@@ -253,6 +324,9 @@ def main():
         'kernels': kernels,
         'cpu_baseline': cpu,
         'rel_l2_vs_fp64': acc,
+        'plain_form': plain,
+        'mfbd_frames_per_s': mfbd['1024']['frames_per_s'] if mfbd else None,
+        'mfbd': mfbd,
     }
     print(json.dumps(out))
 

@@ -77,6 +77,19 @@ int vmbd_ew(float* out, const float* a, const float* b, float s, int64_t n, int3
 /* out = a * b elementwise. */
 int vmbd_mul(float* out, const float* a, const float* b, int64_t n, vmbd_stream_t stream);
 
+/* Isotropic eps-smoothed total variation of K images H x W (the TV term of
+ * the autograd loss; no counterpart in the reference, which has no TV prior):
+ * TV = sum sqrt(dx^2 + dy^2 + eps^2) with forward differences, zero past the
+ * last row/column.  value_out: one device double; partials:
+ * vmbd_reduce_blocks(K*H*W) doubles. */
+int vmbd_tv(const float* x, int32_t K, int32_t H, int32_t W, float eps, double* partials, double* value_out,
+            vmbd_stream_t stream);
+
+/* grad = (accumulate ? grad : 0) + weight * scale[0] * dTV/dx; scale is a
+ * device float (the upstream gradient) or NULL for 1. */
+int vmbd_tv_grad(float* grad, const float* x, int32_t K, int32_t H, int32_t W, float eps, const float* scale,
+                 float weight, int32_t accumulate, vmbd_stream_t stream);
+
 /* Multiplicative latent-image step (vmbd_chimney.py:143-147):
  * x *= max(num / (den + tol), tol); then x = min(x, max of x over rows
  * [border0, H - border0) and columns [border1, W - border1)).  partial_max:

@@ -256,6 +256,62 @@ __global__ __launch_bounds__(TB) void k_rl_clip(float* x, long long n, const flo
     x[e] = x[e] > m ? m : x[e];
 }
 
+// Isotropic, eps-smoothed total variation of K images (H x W):
+//   TV = sum_ij n_ij,  n_ij = sqrt(dx_ij^2 + dy_ij^2 + eps^2),
+//   dx_ij = x[i, j+1] - x[i, j] (0 in the last column),
+//   dy_ij = x[i+1, j] - x[i, j] (0 in the last row).
+__device__ __forceinline__ float tv_norm(const float* s, int H, int W, int i, int j, float eps2, float* dx,
+                                         float* dy) {
+  const float c = s[(long long)i * W + j];
+  const float a = (j + 1 < W) ? s[(long long)i * W + j + 1] - c : 0.f;
+  const float b = (i + 1 < H) ? s[(long long)(i + 1) * W + j] - c : 0.f;
+  *dx = a;
+  *dy = b;
+  return sqrtf(a * a + b * b + eps2);
+}
+
+__global__ __launch_bounds__(TB) void k_tv_value(const float* x, long long n, int H, int W, float eps2,
+                                                 double* partials) {
+  double v[1] = {0.0};
+  const long long HW = (long long)H * W;
+  for (long long e = (long long)blockIdx.x * TB + threadIdx.x; e < n; e += (long long)gridDim.x * TB) {
+    const long long k = e / HW;
+    const int r = (int)(e - k * HW);
+    const int i = r / W, j = r - (r / W) * W;
+    float a, b;
+    v[0] += (double)tv_norm(x + k * HW, H, W, i, j, eps2, &a, &b);
+  }
+  block_sum<1>(v);
+  if (threadIdx.x == 0) partials[blockIdx.x] = v[0];
+}
+
+// grad (+)= weight * scale[0] * dTV/dx:
+//   dTV/dx_ij = -(dx_ij + dy_ij)/n_ij + dx_{i,j-1}/n_{i,j-1} + dy_{i-1,j}/n_{i-1,j}
+__global__ __launch_bounds__(TB) void k_tv_grad(float* grad, const float* x, long long n, int H, int W, float eps2,
+                                                const float* scale, float weight, int accumulate) {
+  const float w = weight * (scale ? scale[0] : 1.f);
+  const long long HW = (long long)H * W;
+  for (long long e = (long long)blockIdx.x * TB + threadIdx.x; e < n; e += (long long)gridDim.x * TB) {
+    const long long k = e / HW;
+    const int r = (int)(e - k * HW);
+    const int i = r / W, j = r - (r / W) * W;
+    const float* s = x + k * HW;
+    float a, b;
+    const float nc = tv_norm(s, H, W, i, j, eps2, &a, &b);
+    float gv = -(a + b) / nc;
+    if (j > 0) {
+      const float nl = tv_norm(s, H, W, i, j - 1, eps2, &a, &b);
+      gv += a / nl;
+    }
+    if (i > 0) {
+      const float nu = tv_norm(s, H, W, i - 1, j, eps2, &a, &b);
+      gv += b / nu;
+    }
+    gv *= w;
+    grad[e] = accumulate ? grad[e] + gv : gv;
+  }
+}
+
 }  // namespace
 }  // namespace vmbd
 
@@ -373,6 +429,31 @@ int vmbd_ew(float* out, const float* a, const float* b, float s, int64_t n, int3
   ProfScope ps("ew", st);
   hipLaunchKernelGGL(k_ew, dim3(grid_for(n)), dim3(TB), 0, st, out, a, b, s, (long long)n, op);
   SOLVER_DONE("vmbd_ew");
+}
+
+int vmbd_tv(const float* x, int32_t K, int32_t H, int32_t W, float eps, double* partials, double* value_out,
+            vmbd_stream_t stream) {
+  vmbd_clear_error();
+  if (!x || !partials || !value_out || K <= 0 || H <= 0 || W <= 0) return vmbd_fail(VMBD_E_ARG, "vmbd_tv: bad argument");
+  hipStream_t st = (hipStream_t)stream;
+  const long long n = (long long)K * H * W;
+  const int nb = grid_for(n);
+  ProfScope ps("tv", st);
+  hipLaunchKernelGGL(k_tv_value, dim3(nb), dim3(TB), 0, st, x, n, H, W, eps * eps, partials);
+  hipLaunchKernelGGL(k_reduce_out, dim3(1), dim3(TB), 0, st, partials, nb, 1, value_out);
+  SOLVER_DONE("vmbd_tv");
+}
+
+int vmbd_tv_grad(float* grad, const float* x, int32_t K, int32_t H, int32_t W, float eps, const float* scale,
+                 float weight, int32_t accumulate, vmbd_stream_t stream) {
+  vmbd_clear_error();
+  if (!grad || !x || K <= 0 || H <= 0 || W <= 0) return vmbd_fail(VMBD_E_ARG, "vmbd_tv_grad: bad argument");
+  hipStream_t st = (hipStream_t)stream;
+  const long long n = (long long)K * H * W;
+  ProfScope ps("tv_grad", st);
+  hipLaunchKernelGGL(k_tv_grad, dim3(grid_for(n)), dim3(TB), 0, st, grad, x, n, H, W, eps * eps, scale, weight,
+                     accumulate);
+  SOLVER_DONE("vmbd_tv_grad");
 }
 
 int vmbd_rl_update(float* x, const float* num, const float* den, int32_t H, int32_t W, float tol, int32_t border0,
