@@ -83,6 +83,8 @@ __global__ __launch_bounds__(WG) void k_rows_r2c(Geo g, RowJobs jobs, const floa
 
   Twiddles<TW> T;
   twiddles_init<TW>(T, t, tw);
+  // a wavefront whose row pairs all lie past the end skips its transforms
+  const bool wact = q0 + (tid >> 6) * (64 / TPT) < NQ;
 
   // rows r0 (real part) and r0+1 (imaginary part)
   const int i0 = o0 + r0, i1 = i0 + 1;
@@ -107,20 +109,25 @@ __global__ __launch_bounds__(WG) void k_rows_r2c(Geo g, RowJobs jobs, const floa
   }
 
   float2* sl = sm + tr * SL;
-  fft<TW, -1>(v, sl, t, T, tw);
-  // natural-order spectrum of z = a + i b into the slice, then split
-#pragma unroll
-  for (int mm = 0; mm < 16; ++mm) sl[lds_pad(t + TPT * mm)] = v[mm];
-  wave_sync();
   float2 za[9], zb[9];
+  if (wact) {
+    fft<TW, -1>(v, sl, t, T, tw);
+    // natural-order spectrum of z = a + i b into the slice, then split
 #pragma unroll
-  for (int mm = 0; mm <= 8; ++mm) {
-    const int c = t + TPT * mm;
-    const float2 zc = sl[lds_pad(c < NBN ? c : 0)];
-    const float2 zm = sl[lds_pad((TW - c) & (TW - 1))];
-    // A = (Z[c] + conj Z[-c]) / 2,  B = (Z[c] - conj Z[-c]) / (2i)
-    za[mm] = make_float2(0.5f * (zc.x + zm.x), 0.5f * (zc.y - zm.y));
-    zb[mm] = make_float2(0.5f * (zc.y + zm.y), 0.5f * (zm.x - zc.x));
+    for (int mm = 0; mm < 16; ++mm) sl[lds_pad(t + TPT * mm)] = v[mm];
+    wave_sync();
+#pragma unroll
+    for (int mm = 0; mm <= 8; ++mm) {
+      const int c = t + TPT * mm;
+      const float2 zc = sl[lds_pad(c < NBN ? c : 0)];
+      const float2 zm = sl[lds_pad((TW - c) & (TW - 1))];
+      // A = (Z[c] + conj Z[-c]) / 2,  B = (Z[c] - conj Z[-c]) / (2i)
+      za[mm] = make_float2(0.5f * (zc.x + zm.x), 0.5f * (zc.y - zm.y));
+      zb[mm] = make_float2(0.5f * (zc.y + zm.y), 0.5f * (zm.x - zc.x));
+    }
+  } else {
+#pragma unroll
+    for (int mm = 0; mm <= 8; ++mm) za[mm] = zb[mm] = make_float2(0.f, 0.f);
   }
   __syncthreads();  // every slice has been read: the buffer becomes staging
 #pragma unroll
@@ -244,6 +251,8 @@ __global__ __launch_bounds__(WG) void k_cols(Geo g, ColIn a, ColIn b, ColIn c3, 
   const int t = threadIdx.x - cc * TPT;
   const int c = strip * CS + cc;
   const bool active = c < NB;
+  // a wavefront whose columns all lie past the last bin skips its transforms
+  const bool wact = strip * CS + (int)(threadIdx.x >> 6) * (64 / TPT) < NB;
   float2* sl = sm + cc * SL;
 
   Twiddles<TH> T;
@@ -251,7 +260,7 @@ __global__ __launch_bounds__(WG) void k_cols(Geo g, ColIn a, ColIn b, ColIn c3, 
 
   float2 A[16];
   col_load<TH>(A, a, p, c, NB, t, active);
-  if (!a.spec) fft<TH, -1>(A, sl, t, T, tw);
+  if (!a.spec && wact) fft<TH, -1>(A, sl, t, T, tw);
   if constexpr (OP == COL_SPEC) {
     if (active) {
       float2* col = spec_out + ((size_t)p * NB + c) * TH;
@@ -261,18 +270,18 @@ __global__ __launch_bounds__(WG) void k_cols(Geo g, ColIn a, ColIn b, ColIn c3, 
   } else {
     float2 B[16];
     col_load<TH>(B, b, p, c, NB, t, active);
-    if (!b.spec) fft<TH, -1>(B, sl, t, T, tw);
+    if (!b.spec && wact) fft<TH, -1>(B, sl, t, T, tw);
 #pragma unroll
     for (int m = 0; m < 16; ++m)
       B[m] = cscale((OP == COL_FWD) ? cmul(A[m], B[m]) : cmulc(A[m], B[m]), scale);
-    fft<TH, +1>(B, sl, t, T, tw);
+    if (wact) fft<TH, +1>(B, sl, t, T, tw);
     col_store_rows<TH, CS>(B, sm, o1, p, strip, NB, cc, t);
     if constexpr (OP == COL_ADJ2) {
       col_load<TH>(B, c3, p, c, NB, t, active);
-      if (!c3.spec) fft<TH, -1>(B, sl, t, T, tw);
+      if (!c3.spec && wact) fft<TH, -1>(B, sl, t, T, tw);
 #pragma unroll
       for (int m = 0; m < 16; ++m) B[m] = cscale(cmulc(A[m], B[m]), scale);
-      fft<TH, +1>(B, sl, t, T, tw);
+      if (wact) fft<TH, +1>(B, sl, t, T, tw);
       col_store_rows<TH, CS>(B, sm, o2, p, strip, NB, cc, t);
     }
   }
@@ -401,8 +410,15 @@ __global__ __launch_bounds__(WG) void k_rows_c2r_ola(Geo g, OlaDst d, const floa
     const float* wa = wrow ? d.win + ((size_t)d.win_idx[ka - kf] * g.sw0 + r) * g.sw1 : nullptr;
     const float* wbp = (wrow && hasb) ? d.win + ((size_t)d.win_idx[ka + 1 - kf] * g.sw0 + r) * g.sw1 : wa;
     float2 v[16];
-    c2r_load<TW>(v, ra, rb, rok && hasb, t);
-    fft<TW, +1>(v, sl, t, T, tw);
+    // a wavefront with no item in this round skips its transforms
+    const bool wact = base + (int)(threadIdx.x >> 6) * (64 / TPT) < NT;
+    if (wact) {
+      c2r_load<TW>(v, ra, rb, rok && hasb, t);
+      fft<TW, +1>(v, sl, t, T, tw);
+    } else {
+#pragma unroll
+      for (int mm = 0; mm < 16; ++mm) v[mm] = make_float2(0.f, 0.f);
+    }
     if (d.alias) __syncthreads();  // slices are about to be overwritten by rowbuf
     if (act) {
       float* ba = rowbuf + (mi * g.csf1 + na) * d.ext1;
@@ -502,6 +518,8 @@ __global__ __launch_bounds__(WG) void k_rows_c2r_stack(Geo g, const float2* __re
   const int NQ = (nrows + 1) >> 1;
   const int q = blockIdx.y * TPW + tr;
   const bool act = q < NQ;
+  // no workgroup barrier follows: a wavefront without row pairs is done
+  if (blockIdx.y * TPW + (int)(threadIdx.x >> 6) * (64 / TPT) >= NQ) return;
   const int p = blockIdx.x;
   const int r0 = 2 * q, r1 = 2 * q + 1;
   const float2* ra = act ? src + ((size_t)p * srow + r0) * g.NB : src;
