@@ -71,9 +71,9 @@ def rl_step(x, f, y, winaux, sf, tol=1e-10, exact=False):
     return np.minimum(x, xmax)
 
 
-def f_step_pbb(x, y, winaux, sf, u, iters, beta, exact=False):
+def f_step_pbb(x, y, winaux, sf, u, iters, beta, exact=False, f0=None):
     """PSF estimate: PBB on 1/2 |X f - y|^2 + beta/2 |f|^2 over f >= 0 from
-    X'y (olaGPU.py:388-392), then normalize."""
+    f0 (X'y when None, olaGPU.py:388-392), then normalize."""
     x = np.asarray(x, dtype=np.float64)
     y = np.asarray(y, dtype=np.float64)
     X = O.OlaOracle(x, sf, 'valid', winaux)
@@ -85,7 +85,7 @@ def f_step_pbb(x, y, winaux, sf, u, iters, beta, exact=False):
     def grad(z):
         return X.cnvtp(X.cnv(z) - y, exact=exact) + beta * z
 
-    z0 = X.cnvtp(y, exact=exact)
+    z0 = X.cnvtp(y, exact=exact) if f0 is None else np.asarray(f0, dtype=np.float64)
     return normalize(pbb_fixed(grad, obj, z0, u, iters))
 
 
@@ -106,9 +106,11 @@ def x_step_pbb(x, f, y, winaux, u, iters, exact=False):
     return pbb_fixed(grad, obj, x, u, iters)
 
 
-def frame_step(x, y, winaux, sf, fu, f_iters, beta, tol=1e-10, exact=False, x_solver='rl', xu=None, x_iters=0):
-    """One frame of the online loop with the device f-solver (PBB)."""
-    f = f_step_pbb(x, y, winaux, sf, fu, f_iters, beta, exact)
+def frame_step(x, y, winaux, sf, fu, f_iters, beta, tol=1e-10, exact=False, x_solver='rl', xu=None, x_iters=0,
+               f0=None):
+    """One frame of the online loop with the device f-solver (PBB) started
+    from f0 (X'y when None)."""
+    f = f_step_pbb(x, y, winaux, sf, fu, f_iters, beta, exact, f0)
     if x_solver == 'rl':
         x = rl_step(x, f, y, winaux, sf, tol, exact)
     else:

@@ -44,23 +44,29 @@ def _scene(seed, sy, sf, csf):
     return sx, wa, y.astype(np.float32).astype(np.float64), y0.astype(np.float32).astype(np.float64)
 
 
-@pytest.mark.parametrize('exact', [False, True])
-def test_frame_pbb_rl_vs_oracle(cuda, exact):
+@pytest.mark.parametrize('exact,f_init', [(False, 'xty'), (True, 'xty'), (False, 'warm')])
+def test_frame_pbb_rl_vs_oracle(cuda, exact, f_init):
     from vmbd_amd.mfbd import OnlineMFBD
     sy, sf, csf = (64, 64), (7, 7), (3, 3)
-    print(f'case mfbd pbb+rl exact={exact}', flush=True)
+    print(f'case mfbd pbb+rl exact={exact} f_init={f_init}', flush=True)
     sx, wa, y, y0 = _scene(11, sy, sf, csf)
-    m = OnlineMFBD(y0, sf, csf, f_solver='pbb', x_solver='rl', f_iters=10, f_beta=0.1, exact=exact)
+    m = OnlineMFBD(y0, sf, csf, f_solver='pbb', x_solver='rl', f_iters=10, f_beta=0.1, exact=exact, f_init=f_init)
     x0 = _n(m.x)
     np.testing.assert_array_equal(x0, np.pad(y0.astype(np.float32), ((3, 3), (3, 3)), mode='edge'))
-    x1, f1 = m.step(y)
-    fu = _n(m.fu)
-    xr, fr = M.frame_step(x0, y, wa, sf, fu, 10, 0.1, tol=1e-10, exact=exact)
-    assert rel_l2(_n(f1), fr) < TOL_ITER
-    assert rel_l2(_n(x1), xr) < TOL_ITER
-    # the x-step alone, from the PSFs the device produced: one pass, operator tolerance
-    xr2 = M.rl_step(x0, _n(f1), y, wa, sf, tol=1e-10, exact=exact)
-    assert rel_l2(_n(x1), xr2) < 1e-5
+    f0 = None if f_init == 'xty' else np.full((9,) + sf, 1.0 / 49, dtype=np.float32).astype(np.float64)
+    rng = np.random.default_rng(15)
+    for frame in range(2 if f_init == 'warm' else 1):
+        yi = y if frame == 0 else y + 0.01 * rng.standard_normal(y.shape)
+        yi = yi.astype(np.float32).astype(np.float64)
+        x1, f1 = m.step(yi)
+        fu = _n(m.fu)
+        xr, fr = M.frame_step(x0, yi, wa, sf, fu, 10, 0.1, tol=1e-10, exact=exact, f0=f0)
+        assert rel_l2(_n(f1), fr) < TOL_ITER, frame
+        assert rel_l2(_n(x1), xr) < TOL_ITER, frame
+        # the x-step alone, from the PSFs the device produced: one pass, operator tolerance
+        xr2 = M.rl_step(x0, _n(f1), yi, wa, sf, tol=1e-10, exact=exact)
+        assert rel_l2(_n(x1), xr2) < 1e-5, frame
+        x0, f0 = _n(x1), _n(f1)      # the next frame starts from the device state
 
 
 def test_frame_pbb_pbb_vs_oracle(cuda):
@@ -71,7 +77,9 @@ def test_frame_pbb_pbb_vs_oracle(cuda):
     m = OnlineMFBD(y0, sf, csf, f_solver='pbb', x_solver='pbb', f_iters=8, x_iters=6, f_beta=0.05, exact=True)
     x0 = _n(m.x)
     x1, f1 = m.step(y)
-    xr, fr = M.frame_step(x0, y, wa, sf, _n(m.fu), 8, 0.05, exact=True, x_solver='pbb', xu=_n(m.xu), x_iters=6)
+    f0 = np.full((6,) + sf, np.float32(1.0 / 25)).astype(np.float64)
+    xr, fr = M.frame_step(x0, y, wa, sf, _n(m.fu), 8, 0.05, exact=True, x_solver='pbb', xu=_n(m.xu), x_iters=6,
+                          f0=f0)
     assert rel_l2(_n(f1), fr) < TOL_ITER
     assert rel_l2(_n(x1), xr) < TOL_ITER
 

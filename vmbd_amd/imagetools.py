@@ -164,3 +164,63 @@ def fft_tile(sw, sf, minimum=16):
     s = np.asarray(sw) + np.asarray(sf) - 1
     t = (2 ** np.ceil(np.log2(np.maximum(s, 1)))).astype(np.int64)
     return tuple(int(max(minimum, v)) for v in t)
+
+
+# ------------------------------------------------------------ frame I/O
+def imread(filename):
+    """8-bit PNG frame as float32 in [0, 1] (what matplotlib's imread gives
+    for the reference's frames, vmbd_chimney.py:55); colour images are
+    reduced to their green channel (rgb2gray, imagetools.py:266-287)."""
+    from PIL import Image
+    a = np.asarray(Image.open(filename))
+    if a.ndim > 2:
+        a = a[..., 1]
+    if a.dtype == np.uint8:
+        return a.astype(np.float32) / np.float32(255.0)
+    if a.dtype == np.uint16:
+        return a.astype(np.float32) / np.float32(65535.0)
+    return a.astype(np.float32)
+
+
+def imwrite(x, filename):
+    """imagetools.py:290-311: an image in [0, 1] written as 8 bits (x * 255
+    truncated to an integer; values outside [0, 1] are clipped, where the
+    reference's uint8 conversion would wrap them)."""
+    from PIL import Image
+    t = np.array(x, dtype=np.float64, copy=True)
+    if t.dtype != np.uint8:
+        t *= 255.0
+    Image.fromarray(np.uint8(np.clip(t, 0, 255))).save(filename)
+
+
+def gridF(fs, csf):
+    """imagetools.py:211-238: the PSF stack (P, sf0, sf1) laid out as one
+    image of csf0 x csf1 tiles, patch k = m*csf1 + n at tile (m, n) (the
+    reference's placement for the square grids it supports)."""
+    fs = np.asarray(fs)
+    c0, c1 = int(csf[0]), int(csf[1])
+    h, w = fs.shape[1], fs.shape[2]
+    f = np.ones((h * c0, w * c1))
+    for i in range(c0 * c1):
+        k, l = divmod(i, c1)
+        f[k * h:(k + 1) * h, l * w:(l + 1) * w] = fs[i]
+    return f
+
+
+def randomwalk(sf, N=None, rng=None):
+    """imagetools.py:99-132: a random walk of N steps (default prod(sf)/100)
+    from the centre, each step moving every coordinate by -1, 0 or +1,
+    clipped to [1, min(sf)-1]; normalised to sum 1.  `rng` is a numpy
+    Generator (the reference uses pylab's global generator)."""
+    rng = np.random.default_rng() if rng is None else rng
+    sf = tuple(int(v) for v in np.atleast_1d(sf))
+    if N is None:
+        N = int(np.floor(np.prod(sf) / 100))
+    w = np.zeros(sf)
+    loc = np.ceil(np.array(sf) / 2.0)
+    w[tuple(loc.astype(int))] = 1.0
+    for _ in range(int(N)):
+        loc = loc + ((rng.random(len(sf)) - 0.5) * 2).round()
+        loc = np.clip(loc, 1, min(sf) - 1)
+        w[tuple(loc.astype(int))] += 1
+    return w / w.sum()

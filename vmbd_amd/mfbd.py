@@ -14,8 +14,12 @@ Solvers:
 * ``f_solver='lbfgsb'``: host L-BFGS-B through ``OlaGPU.deconv`` (the
   reference's, started from X'y, up to ``optiter`` evaluations);
 * ``f_solver='pbb'``: projected Barzilai-Borwein on the device for a fixed
-  ``f_iters`` iterations, started like optGPU.PBB (optGPU.py:256-297) from
-  X'y and a seeded random second point;
+  ``f_iters`` iterations, started like optGPU.PBB (optGPU.py:256-297) from a
+  first point and a seeded random second point.  The first point is the
+  previous frame's PSFs (``f_init='warm'``, flat 1/(sf0 sf1) PSFs on the
+  first frame, the reference's commented-out alternative at olaGPU.py:393)
+  or X'y (``f_init='xty'``, the L-BFGS-B start of olaGPU.py:388-392, whose
+  scale (~1e6 per PSF on 8-bit frames) a few PBB iterations do not undo);
 * ``x_solver='rl'``: the multiplicative step above (the reference's);
 * ``x_solver='pbb'``: ``x_iters`` PBB iterations on 1/2 |mask (F x - y)|^2
   over x >= 0.
@@ -63,13 +67,16 @@ class OnlineMFBD:
 
     def __init__(self, y0, sf, csf=(3, 3), overlap=0.5, f_solver='lbfgsb', x_solver='rl', f_alpha=0.,
                  f_beta=0.1, optiter=500, tol=1e-10, f_iters=20, x_iters=20, exact=False, seed=120, capture=False,
-                 device=None):
+                 f_init='warm', device=None):
         if f_solver not in ('lbfgsb', 'pbb'):
             raise ValueError("f_solver must be 'lbfgsb' or 'pbb'")
         if x_solver not in ('rl', 'pbb'):
             raise ValueError("x_solver must be 'rl' or 'pbb'")
         if capture and f_solver != 'pbb':
             raise ValueError('capture needs the device f-step (f_solver=pbb)')
+        if f_init not in ('warm', 'xty'):
+            raise ValueError("f_init must be 'warm' or 'xty'")
+        self.f_init = f_init
         self.device = torch.device(device) if device is not None else torch.device('cuda', torch.cuda.current_device())
         y0 = gputools._dev(y0)
         self.sy = tuple(int(v) for v in y0.shape)
@@ -122,7 +129,8 @@ class OnlineMFBD:
         self.den = torch.zeros(self.sx, dtype=f32, device=d)
         self.part_max = torch.zeros(_lib.lib().vmbd_reduce_blocks(nx), dtype=f32, device=d)
         self.xmax = torch.zeros(1, dtype=f32, device=d)
-        self.f_dev = torch.zeros(fshape, dtype=f32, device=d)
+        # PSF state: flat PSFs until the first frame has been processed
+        self.f_dev = torch.full(fshape, 1.0 / (self.sf[0] * self.sf[1]), dtype=f32, device=d)
 
     # ---------------------------------------------------------- primitives
     def _ew(self, op, out, a, b=None, s=0.0):
@@ -178,8 +186,10 @@ class OnlineMFBD:
         p = self.plan
         p.spectra('X', self.x, out=self.specX)
         z, oldz, g, oldg = self.fz
-        # start point X'y (olaGPU.py:388-392)
-        p.adjoint_cached('X', self.specX, self.y, exact=self.exact, out=z)
+        if self.f_init == 'warm':
+            z.copy_(self.f_dev)
+        else:   # X'y (olaGPU.py:388-392)
+            p.adjoint_cached('X', self.specX, self.y, exact=self.exact, out=z)
         res = self._pbb(self._grad_f, z, oldz, g, oldg, self.fu, self.f_iters, self.f_beta, self.sc_f)
         self.f_dev.copy_(res)
         _lib.call('vmbd_psf_normalize', _lib.ptr(self.f_dev), self.P, self.sf[0] * self.sf[1], 1e-6,
@@ -246,15 +256,17 @@ class OnlineMFBD:
 
     def _record(self):
         # one eager frame on a side stream initialises every lazy buffer, then
-        # the same launches are recorded; the latent image is restored so the
-        # recorded frame is the first one processed
+        # the same launches are recorded; the latent image and the PSF state
+        # are restored so the recorded frame is the first one processed
         x0 = self.x.clone()
+        f0 = self.f_dev.clone()
         s = torch.cuda.Stream(device=self.device)
         s.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(s):
             self._frame_body()
         torch.cuda.current_stream().wait_stream(s)
         self.x.copy_(x0)
+        self.f_dev.copy_(f0)
         g = torch.cuda.CUDAGraph()
         with torch.cuda.graph(g):
             self._frame_body()
