@@ -105,10 +105,10 @@ def plain_form(wa, x, f, g, timed, a):
                     'operation, complex (T x T) spectra, patch-local transposes'}
 
 
-def mfbd_rate(sx, csf, sf, walk, nframes, timed):
+def mfbd_rate(sx, csf, sf, walk, nframes, timed, iters=20, prior=None):
     """Online MFBD frames/s: latent sx, PSF grid csf of sf PSFs, per frame
-    20 PBB f-iterations + 20 PBB x-iterations (exact adjoints), captured in
-    a HIP graph and replayed per frame."""
+    `iters` PBB f-iterations + `iters` PBB x-iterations (exact adjoints) and
+    the optional prior pass, captured in a HIP graph and replayed per frame."""
     import numpy as np
     import torch
 
@@ -128,8 +128,8 @@ def mfbd_rate(sx, csf, sf, walk, nframes, timed):
         yt = plan.forward(xt, ft)
         noise = torch.as_tensor((0.01 * rng.standard_normal(plan.sy)).astype(np.float32), device=dev)
         obs.append(yt + noise)
-    m = OnlineMFBD(obs[0], sf, csf, f_solver='pbb', x_solver='pbb', f_iters=20, x_iters=20, f_beta=1e-3,
-                   exact=True, capture=True, device=dev)
+    m = OnlineMFBD(obs[0], sf, csf, f_solver='pbb', x_solver='pbb', f_iters=iters, x_iters=iters, f_beta=1e-3,
+                   exact=True, capture=True, prior=prior, device=dev)
     for i in range(2):
         m.step(obs[i % 4])
     k = [0]
@@ -140,8 +140,8 @@ def mfbd_rate(sx, csf, sf, walk, nframes, timed):
 
     ms = timed(one, nframes)
     return {'frames_per_s': round(1e3 / ms, 2), 'ms_per_frame': round(ms, 3), 'frames': nframes,
-            'latent': list(sx), 'csf': list(csf), 'sf': list(sf), 'f_iters': 20, 'x_iters': 20,
-            'graph': True}
+            'latent': list(sx), 'csf': list(csf), 'sf': list(sf), 'tile': list(plan.tile), 'f_iters': iters,
+            'x_iters': iters, 'prior': prior, 'graph': True}
 
 
 def main():
@@ -264,7 +264,9 @@ def main():
     mfbd = None
     if not a.no_mfbd:
         mfbd = {'1024': mfbd_rate((H, W), CSF, SF, 40, a.mfbd_frames, timed),
-                '512': mfbd_rate((512, 512), (8, 8), (31, 31), 15, a.mfbd_frames, timed)}
+                '512': mfbd_rate((512, 512), (8, 8), (31, 31), 15, a.mfbd_frames, timed),
+                '2048_prior': mfbd_rate((2048, 2048), (16, 16), (63, 63), 60, a.mfbd_frames, timed, iters=10,
+                                        prior='bilateral_shock')}
 
     cpu = None
     acc = None

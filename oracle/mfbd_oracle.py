@@ -8,6 +8,7 @@ device path and here), so the iterate sequence is deterministic.
 """
 import numpy as np
 
+from . import filters_oracle as FO
 from . import ola_oracle as O
 
 
@@ -107,12 +108,15 @@ def x_step_pbb(x, f, y, winaux, u, iters, exact=False):
 
 
 def frame_step(x, y, winaux, sf, fu, f_iters, beta, tol=1e-10, exact=False, x_solver='rl', xu=None, x_iters=0,
-               f0=None):
+               f0=None, prior=False):
     """One frame of the online loop with the device f-solver (PBB) started
-    from f0 (X'y when None)."""
+    from f0 (X'y when None); `prior`: bilateral (5, 3, 0.1) then three shock
+    steps (dt 0.1) on the latent image."""
     f = f_step_pbb(x, y, winaux, sf, fu, f_iters, beta, exact, f0)
     if x_solver == 'rl':
         x = rl_step(x, f, y, winaux, sf, tol, exact)
     else:
         x = x_step_pbb(x, f, y, winaux, xu, x_iters, exact)
+    if prior:
+        x = FO.shock_filter(FO.bilateral(x, 5, 3.0, 0.1), 3, 0.1, 1.0)
     return x, f

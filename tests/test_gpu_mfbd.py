@@ -84,6 +84,23 @@ def test_frame_pbb_pbb_vs_oracle(cuda):
     assert rel_l2(_n(x1), xr) < TOL_ITER
 
 
+def test_frame_with_prior_vs_oracle(cuda):
+    """PBB f-step, PBB x-step and the bilateral + shock prior pass."""
+    from vmbd_amd.mfbd import OnlineMFBD
+    sy, sf, csf = (64, 64), (7, 7), (3, 3)
+    print('case mfbd prior', flush=True)
+    sx, wa, y, y0 = _scene(16, sy, sf, csf)
+    m = OnlineMFBD(y0, sf, csf, f_solver='pbb', x_solver='pbb', f_iters=6, x_iters=4, f_beta=0.1, exact=True,
+                   prior='bilateral_shock')
+    x0 = _n(m.x)
+    x1, f1 = m.step(y)
+    f0 = np.full((9,) + sf, np.float32(1.0 / 49)).astype(np.float64)
+    xr, fr = M.frame_step(x0, y, wa, sf, _n(m.fu), 6, 0.1, exact=True, x_solver='pbb', xu=_n(m.xu), x_iters=4,
+                          f0=f0, prior=True)
+    assert rel_l2(_n(f1), fr) < TOL_ITER
+    assert rel_l2(_n(x1), xr) < TOL_ITER
+
+
 def test_capture_replays_match_eager(cuda):
     """Three frames replayed from one recorded graph equal three eager frames
     bit for bit (deterministic reductions, same random draws)."""

@@ -22,7 +22,10 @@ Solvers:
   scale (~1e6 per PSF on 8-bit frames) a few PBB iterations do not undo);
 * ``x_solver='rl'``: the multiplicative step above (the reference's);
 * ``x_solver='pbb'``: ``x_iters`` PBB iterations on 1/2 |mask (F x - y)|^2
-  over x >= 0.
+  over x >= 0;
+* ``prior='bilateral_shock'``: after the x-step, a bilateral filter
+  (half width 5, sigma_d 3, sigma_r 0.1, gputools.py:251-285) and three
+  shock-filter steps (dt 0.1, gputools.py:1282-1314) on the latent image.
 
 With ``f_solver='pbb'`` a frame is a fixed sequence of launches with no host
 synchronisation; ``capture=True`` records it once in a HIP graph and replays
@@ -67,7 +70,7 @@ class OnlineMFBD:
 
     def __init__(self, y0, sf, csf=(3, 3), overlap=0.5, f_solver='lbfgsb', x_solver='rl', f_alpha=0.,
                  f_beta=0.1, optiter=500, tol=1e-10, f_iters=20, x_iters=20, exact=False, seed=120, capture=False,
-                 f_init='warm', device=None):
+                 f_init='warm', prior=None, device=None):
         if f_solver not in ('lbfgsb', 'pbb'):
             raise ValueError("f_solver must be 'lbfgsb' or 'pbb'")
         if x_solver not in ('rl', 'pbb'):
@@ -77,6 +80,9 @@ class OnlineMFBD:
         if f_init not in ('warm', 'xty'):
             raise ValueError("f_init must be 'warm' or 'xty'")
         self.f_init = f_init
+        if prior not in (None, 'bilateral_shock'):
+            raise ValueError("prior must be None or 'bilateral_shock'")
+        self.prior = prior
         self.device = torch.device(device) if device is not None else torch.device('cuda', torch.cuda.current_device())
         y0 = gputools._dev(y0)
         self.sy = tuple(int(v) for v in y0.shape)
@@ -125,6 +131,7 @@ class OnlineMFBD:
         self.fu = torch.zeros(fshape, dtype=f32, device=d)
         self.xz = [torch.zeros(self.sx, dtype=f32, device=d) for _ in range(3)]    # oldx, g, oldg
         self.xu = torch.zeros(self.sx, dtype=f32, device=d)
+        self.tmp = [torch.zeros(self.sx, dtype=f32, device=d) for _ in range(2)]
         self.num = torch.zeros(self.sx, dtype=f32, device=d)
         self.den = torch.zeros(self.sx, dtype=f32, device=d)
         self.part_max = torch.zeros(_lib.lib().vmbd_reduce_blocks(nx), dtype=f32, device=d)
@@ -220,6 +227,16 @@ class OnlineMFBD:
             if res is not self.x:
                 self.x.copy_(res)
 
+    def _prior(self):
+        """Bilateral filter, then three shock steps, ping-ponging between two
+        buffers so that the result lands in x (no allocation)."""
+        H, W = self.sx
+        st = _lib.stream_ptr()
+        t1, t2 = self.tmp
+        _lib.call('vmbd_bilateral', _lib.ptr(t1), _lib.ptr(self.x), H, W, 5, 3.0, 0.1, st)
+        for src, dst in ((t1, t2), (t2, t1), (t1, self.x)):
+            _lib.call('vmbd_shock_step', _lib.ptr(dst), _lib.ptr(src), H, W, 0.1, 1.0, st)
+
     def _frame_body(self):
         # vmbd_chimney.py:117-118: mask = 1. * (y < 1)
         self._ew(gputools.EW_LT, self.mask, self.y, None, 1.0)
@@ -229,6 +246,8 @@ class OnlineMFBD:
         else:
             self._f_step_lbfgsb()
         self._x_step()
+        if self.prior:
+            self._prior()
 
     def _draw(self):
         torch.rand(self.fu.shape, generator=self.gen, device=self.device, out=self.fu)
