@@ -1,0 +1,96 @@
+"""Mixed-geometry batches (frames of different size, patch grid and PSF
+support in one step) and the tile autotuner, against the FP64 oracle and
+against single-frame plans."""
+import numpy as np
+import pytest
+
+from oracle import ola_oracle as O
+from tests.helpers import case_inputs, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+SPECS = [((96, 96), (3, 3), (9, 9)), ((70, 90), (2, 3), (7, 5)), ((96, 96), (3, 3), (9, 9)),
+         ((120, 100), (3, 4), (6, 8)), ((70, 90), (2, 3), (7, 5))]
+
+
+def _frames(cuda):
+    import torch
+    xs, fs, gs = [], [], []
+    for i, (sx, csf, sf) in enumerate(SPECS):
+        x, f, g = case_inputs(60 + i, sx, csf, sf)
+        t = lambda a: torch.as_tensor(np.asarray(a, dtype=np.float32), device=cuda)
+        xs.append(t(x))
+        fs.append(t(f))
+        gs.append(t(g))
+    return xs, fs, gs
+
+
+def test_mixed_batch_vs_oracle_and_single_plans(cuda):
+    import torch
+    from vmbd_amd.batch import MixedBatch
+    from vmbd_amd.imagetools import win2winaux
+    from vmbd_amd.ola import OlaPlan
+    print('case mixed batch', flush=True)
+    xs, fs, gs = _frames(cuda)
+    mb = MixedBatch(SPECS)
+    assert len(mb.groups) == 3
+    mb.load(xs, fs, gs)
+    mb.step()
+    torch.cuda.synchronize()
+    for i, (sx, csf, sf) in enumerate(SPECS):
+        y, xb, fb = mb.outputs(i)
+        owa = O.Win2WinAux(sx, csf, 0.5)
+        x64, f64, g64 = (t.cpu().numpy().astype(np.float64) for t in (xs[i], fs[i], gs[i]))
+        assert rel_l2(y.cpu().numpy(), O.forward(x64, f64, owa)) < 1e-5, i
+        assert rel_l2(xb.cpu().numpy(), O.adjoint_x(g64, f64, owa, sx, exact=True)) < 1e-5, i
+        assert rel_l2(fb.cpu().numpy(), O.adjoint_f(g64, x64, owa, sf, exact=True)) < 1e-5, i
+        # batching changes nothing: the single-frame plan gives the same bits
+        p1 = OlaPlan(sx, sf, win2winaux(sx, csf, 0.5), 'valid')
+        assert torch.equal(p1.forward(xs[i], fs[i]), y), i
+        xb1, fb1 = p1.adjoint_xf(gs[i], xs[i], fs[i], exact=True)
+        assert torch.equal(xb1, xb) and torch.equal(fb1, fb), i
+
+
+def test_mixed_batch_graph_replay(cuda):
+    import torch
+    from vmbd_amd.batch import MixedBatch
+    print('case mixed batch graph', flush=True)
+    xs, fs, gs = _frames(cuda)
+    mb = MixedBatch(SPECS)
+    mb.load(xs, fs, gs)
+    mb.step()
+    torch.cuda.synchronize()
+    ref = [tuple(t.clone() for t in mb.outputs(i)) for i in range(len(SPECS))]
+    replay = mb.capture()
+    for rep in range(3):
+        for G in mb.groups:
+            G['y'].zero_()
+            G['xb'].zero_()
+            G['fb'].zero_()
+        torch.cuda.synchronize()
+        replay()
+        torch.cuda.synchronize()
+        for i in range(len(SPECS)):
+            for a, b in zip(mb.outputs(i), ref[i]):
+                assert torch.equal(a, b), (rep, i)
+
+
+def test_autotuned_tile_gives_same_result(cuda, tmp_path, monkeypatch):
+    import torch
+    from vmbd_amd import autotune
+    from vmbd_amd.imagetools import win2winaux
+    from vmbd_amd.ola import OlaPlan
+    print('case autotune', flush=True)
+    monkeypatch.setattr(autotune, 'CACHE', str(tmp_path / 'tiles.json'))
+    sx, csf, sf = (96, 96), (3, 3), (9, 9)
+    wa = win2winaux(sx, csf, 0.5)
+    tile, times = autotune.best_tile(sx, sf, wa, frames=2)
+    assert tile in autotune.candidates(np.broadcast_to(wa.sw, (2,)), sf) and len(times) == 4
+    assert tile == autotune.best_tile(sx, sf, wa, frames=2)[0]          # cached
+    x, f, g = case_inputs(70, sx, csf, sf)
+    t = lambda a: torch.as_tensor(np.asarray(a, dtype=np.float32), device=cuda)
+    owa = O.Win2WinAux(sx, csf, 0.5)
+    x64, f64 = (np.asarray(a, np.float32).astype(np.float64) for a in (x, f))
+    for tl in autotune.candidates(np.broadcast_to(wa.sw, (2,)), sf):
+        p = OlaPlan(sx, sf, wa, 'valid', tile=tl)
+        assert rel_l2(p.forward(t(x), t(f)).cpu().numpy(), O.forward(x64, f64, owa)) < 1e-5, tl
