@@ -17,7 +17,7 @@ and the online MFBD loop in frames/s (1024^2 latent, 20 projected-BB
 f-iterations + 20 x-iterations per frame, graph-captured).
 
     python bench.py [--steps K] [--warmup W] [--frames B] [--no-graph] [--cpu-seconds S]
-                    [--no-plain] [--no-mfbd] [--mfbd-frames N]
+                    [--no-plain] [--no-mfbd] [--mfbd-frames N] [--no-mixed]
 """
 import argparse
 import json
@@ -47,6 +47,7 @@ def parse():
     ap.add_argument('--no-plain', action='store_true', help='skip the plain per-operator comparator')
     ap.add_argument('--no-mfbd', action='store_true', help='skip the MFBD frames/s measurement')
     ap.add_argument('--mfbd-frames', type=int, default=10, help='timed MFBD frames')
+    ap.add_argument('--no-mixed', action='store_true', help='skip the 64-frame mixed-geometry batch')
     ap.add_argument('--calib', action='store_true',
                     help='run one 64 MiB elementwise copy first (counter calibration)')
     return ap.parse_args()
@@ -90,6 +91,24 @@ def operator_bytes():
     Sy = (H - SF[0] + 1) * (W - SF[1] + 1)
     Pf = CSF[0] * CSF[1] * SF[0] * SF[1]
     return 4.0 * 3 * (Sx + Sy + Pf)
+
+
+TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'profiles', 'ola_c3_b8_hbm_traffic.md')
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` at this workload from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE x its calibration factor +
+    WRITE_SIZE), or None when the summary has no line for it."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            for line in fh:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == 'traffic_bytes_per_launch' and parts[1] == kernel:
+                    return int(parts[2])
+    except OSError:
+        pass
+    return None
 
 
 def plain_form(wa, x, f, g, timed, a):
@@ -142,6 +161,40 @@ def mfbd_rate(sx, csf, sf, walk, nframes, timed, iters=20, prior=None):
     return {'frames_per_s': round(1e3 / ms, 2), 'ms_per_frame': round(ms, 3), 'frames': nframes,
             'latent': list(sx), 'csf': list(csf), 'sf': list(sf), 'tile': list(plan.tile), 'f_iters': iters,
             'x_iters': iters, 'prior': prior, 'graph': True}
+
+
+C4_GEOMS = [((512, 512), (12, 12), (15, 15)), ((512, 512), (6, 6), (41, 41)), ((768, 1024), (8, 8), (25, 25)),
+            ((768, 1024), (6, 6), (15, 15)), ((1024, 1024), (10, 10), (41, 41)), ((1024, 1024), (12, 12), (25, 25))]
+
+
+def mixed_rate(nframes, timed, steps):
+    """64 frames of six geometries (512^2 / 768x1024 / 1024^2, 6x6..12x12
+    patches, 15/25/41 PSFs): forward + both exact adjoints of every frame,
+    one graph replay per step."""
+    import numpy as np
+    import torch
+
+    from vmbd_amd import synth
+    from vmbd_amd.batch import MixedBatch
+
+    dev = torch.device('cuda', 0)
+    rng = np.random.default_rng(7)
+    specs = [C4_GEOMS[i % len(C4_GEOMS)] for i in range(nframes)]
+    mb = MixedBatch(specs, device=dev)
+    to = lambda v: torch.as_tensor(np.ascontiguousarray(v, dtype=np.float32), device=dev)
+    xs, fs, gs = [], [], []
+    for sx, csf, sf in specs:
+        xs.append(to(synth.piecewise_smooth(sx, rng)))
+        fs.append(to(synth.motion_psfs(csf[0] * csf[1], sf, rng, steps=sf[0])))
+        gs.append(to(0.01 * rng.standard_normal((sx[0] - sf[0] + 1, sx[1] - sf[1] + 1))))
+    mb.load(xs, fs, gs)
+    replay = mb.capture()
+    for _ in range(2):
+        replay()
+    ms = timed(replay, steps)
+    return {'frames': nframes, 'geometries': [[list(a), list(b), list(c)] for a, b, c in C4_GEOMS],
+            'groups': len(mb.groups), 'ms_per_step': round(ms, 3), 'frames_per_s': round(nframes / (ms * 1e-3), 1),
+            'mpix_s': round(mb.pixels / (ms * 1e-3) / 1e6, 1), 'graph': True}
 
 
 def main():
@@ -249,7 +302,8 @@ def main():
     dom = max(kernels, key=lambda k: kernels[k]['total_us_per_step'])
     d = kernels[dom]
     roofline = dict(kernel=dom, bound='hbm', achieved=d.get('hbm_gbs', 0.0), peak=HBM_PEAK_GBS, unit='GB/s',
-                    frac=round(d.get('hbm_gbs', 0.0) / HBM_PEAK_GBS, 4), traffic=None, avg_us=d['avg_us'],
+                    frac=round(d.get('hbm_gbs', 0.0) / HBM_PEAK_GBS, 4),
+                    traffic=measured_traffic(dom) if B == 8 else None, avg_us=d['avg_us'],
                     valu=dict(achieved_tflops=d.get('tflops', 0.0), peak_nonpacked=VALU_FP32_PEAK,
                               frac=round(d.get('tflops', 0.0) / VALU_FP32_PEAK, 4)))
     op_bytes = operator_bytes() * B
@@ -260,6 +314,10 @@ def main():
         plain = plain_form(wa, x[0], f[0], g[0], timed, a)
         plain['fused_speedup'] = round(mpix / plain['mpix_s'], 2)
         plain['fused_speedup_single_frame_eager'] = round(H * W / (ms1 * 1e-3) / 1e6 / plain['mpix_s'], 2)
+
+    mixed = None
+    if not a.no_mixed:
+        mixed = mixed_rate(64, timed, max(3, min(a.steps, 10)))
 
     mfbd = None
     if not a.no_mfbd:
@@ -329,6 +387,7 @@ def main():
         'plain_form': plain,
         'mfbd_frames_per_s': mfbd['1024']['frames_per_s'] if mfbd else None,
         'mfbd': mfbd,
+        'mixed_batch': mixed,
     }
     print(json.dumps(out))
 

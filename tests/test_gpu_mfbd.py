@@ -44,6 +44,21 @@ def _scene(seed, sy, sf, csf):
     return sx, wa, y.astype(np.float32).astype(np.float64), y0.astype(np.float32).astype(np.float64)
 
 
+def _rl_rel(x_dev, x_ref, x0, f, y, wa, sf, exact):
+    """Relative L2 of the multiplicative x-step over the pixels it determines.
+
+    A latent pixel that no unmasked observation reaches with nonzero PSF
+    weight has F'(mask y) = F'(mask F x) = 0 exactly; its update is float32
+    round-off over round-off (0/0, in the reference as well), so such pixels
+    (den below 1e-6 of its maximum) are left out; they must be rare."""
+    F = O.OlaOracle(f, x0.shape, 'valid', wa)
+    mask = (np.asarray(y) < 1).astype(np.float64)
+    den = F.cnvtp(mask * F.cnv(x0), exact=exact)
+    ok = den > 1e-6 * den.max()
+    assert ok.mean() > 0.99
+    return rel_l2(x_dev[ok], x_ref[ok])
+
+
 @pytest.mark.parametrize('exact,f_init', [(False, 'xty'), (True, 'xty'), (False, 'warm')])
 def test_frame_pbb_rl_vs_oracle(cuda, exact, f_init):
     from vmbd_amd.mfbd import OnlineMFBD
@@ -62,10 +77,10 @@ def test_frame_pbb_rl_vs_oracle(cuda, exact, f_init):
         fu = _n(m.fu)
         xr, fr = M.frame_step(x0, yi, wa, sf, fu, 10, 0.1, tol=1e-10, exact=exact, f0=f0)
         assert rel_l2(_n(f1), fr) < TOL_ITER, frame
-        assert rel_l2(_n(x1), xr) < TOL_ITER, frame
+        assert _rl_rel(_n(x1), xr, x0, fr, yi, wa, sf, exact) < TOL_ITER, frame
         # the x-step alone, from the PSFs the device produced: one pass, operator tolerance
         xr2 = M.rl_step(x0, _n(f1), yi, wa, sf, tol=1e-10, exact=exact)
-        assert rel_l2(_n(x1), xr2) < 1e-5, frame
+        assert _rl_rel(_n(x1), xr2, x0, _n(f1), yi, wa, sf, exact) < 1e-5, frame
         x0, f0 = _n(x1), _n(f1)      # the next frame starts from the device state
 
 
@@ -139,7 +154,7 @@ def test_frame_lbfgsb(cuda):
     assert np.all(f1 >= 0)
     np.testing.assert_allclose(f1.sum(axis=(1, 2)), 1.0, rtol=1e-5)
     xr = M.rl_step(x0, f1, y, wa, sf, tol=1e-10)
-    assert rel_l2(_n(x1), xr) < 1e-5
+    assert _rl_rel(_n(x1), xr, x0, f1, y, wa, sf, False) < 1e-5
 
 
 def test_pbb_class_converges_to_nnls(cuda):

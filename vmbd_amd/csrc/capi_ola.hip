@@ -286,6 +286,12 @@ vmbd_plan* vmbd_plan_create(const vmbd_geom* gm, const float* win_table, int32_t
     vmbd_fail(VMBD_E_ARG, "vmbd_plan_create: max_frames=%d outside [0, 65535]", G.max_frames);
     return nullptr;
   }
+  if ((long long)(G.max_frames > 0 ? G.max_frames : 1) * G.csf0 * G.csf1 > 65535) {
+    // the column pass puts (frame, patch) on the second grid dimension
+    vmbd_fail(VMBD_E_ARG, "vmbd_plan_create: max_frames*csf0*csf1=%lld exceeds 65535",
+              (long long)(G.max_frames > 0 ? G.max_frames : 1) * G.csf0 * G.csf1);
+    return nullptr;
+  }
   for (int k = 0; k < g.Pf; ++k) {
     if (win_idx[k] < 0 || win_idx[k] >= n_win) {
       vmbd_fail(VMBD_E_ARG, "vmbd_plan_create: win_idx[%d]=%d outside [0,%d)", k, win_idx[k], n_win);

@@ -245,8 +245,10 @@ __global__ __launch_bounds__(WG) void k_cols(Geo g, ColIn a, ColIn b, ColIn c3, 
   __shared__ float2 sm[(OP == COL_SPEC) ? CS * SL : cmax<CS * SL, TH * (CS + 1)>::v];
 
   const int NB = g.NB;
-  const int p = blockIdx.x;
-  const int strip = blockIdx.y;
+  // strips of one patch are consecutive workgroups: the unaligned 16-bin
+  // segments they write share cache lines, which then fill while resident
+  const int strip = blockIdx.x;
+  const int p = blockIdx.y;
   const int cc = threadIdx.x / TPT;
   const int t = threadIdx.x - cc * TPT;
   const int c = strip * CS + cc;
@@ -292,7 +294,7 @@ static hipError_t cols_t(const Geo& g, int op, const ColIn& a, const ColIn& b, c
                          const ColOut& o2, float scale, float2* spec_out, const float2* tw, hipStream_t st) {
   constexpr int CS = WG / (TH / 16);
   const int strips = (g.NB + CS - 1) / CS;
-  const dim3 grid((unsigned)g.P, (unsigned)strips);
+  const dim3 grid((unsigned)strips, (unsigned)g.P);
   static const char* names[4] = {"cols_spec", "cols_fwd", "cols_adj1", "cols_adj2"};
   ProfScope ps(names[op & 3], st);
   switch (op) {

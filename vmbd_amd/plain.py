@@ -11,7 +11,12 @@ of conj-FFT-conj, which is the same arithmetic).
     p = PlainOla(sx, sf, winaux)
     y, xbar, fbar = p.step(x, f, g)      # F ctor, X ctor, F.cnv, F.cnvtp, X.cnvtp
 
-Valid mode; the transposes are the reference's patch-local ones.
+Valid mode; the transposes are the reference's patch-local ones,
+z = IFFT(G^ . conj(C^)) with the e^{-i} forward transform used here: the
+correlation z[s] = sum_b G[b] c[b - s] that makes X.cnvtp the transpose of
+X.cnv on the patch (olaGPU.py:290-297 writes conj(G^) . C^ for pyfft's
+transform; with an e^{-i} forward transform that product would correlate in
+the opposite direction).
 """
 import numpy as np
 import torch
@@ -66,14 +71,14 @@ class PlainOla:
 
     def cnvtp_F(self, Fh, g):
         """F.cnvtp(g), olaGPU.py:275-311 (valid), OLA over the patch extent sw."""
-        z = self._fft(G.cmul(self._residual_spectra(g), Fh, mode=2), inverse=True)
+        z = self._fft(G.cmul(self._residual_spectra(g), Fh, mode=1), inverse=True)
         zs = G.crop_stack_GPU(z, self.sw)
         zs = G.ew(G.EW_MUL, zs, self.ws)
         return G.crop_gpu2cpu(G.ola_GPU_test(zs, self.csf, self.sw, self.nhop), self.sx)
 
     def cnvtp_X(self, Xh, g):
         """X.cnvtp(g), olaGPU.py:275-302 (valid)."""
-        z = self._fft(G.cmul(self._residual_spectra(g), Xh, mode=2), inverse=True)
+        z = self._fft(G.cmul(self._residual_spectra(g), Xh, mode=1), inverse=True)
         return G.crop_stack_GPU(z, self.sf)
 
     def step(self, x, f, g):
