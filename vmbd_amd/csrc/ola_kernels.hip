@@ -28,6 +28,7 @@
 namespace vmbd {
 
 constexpr int WG = 256;
+constexpr int NXCD = 8;  // XCDs of an MI355X; workgroups are dealt to them round-robin
 
 template <int A, int B>
 struct cmax {
@@ -45,10 +46,18 @@ __global__ __launch_bounds__(WG) void k_rows_r2c(Geo g, RowJobs jobs, const floa
   constexpr int SL = FftLds<TW>::floats2;
   __shared__ float2 sm[cmax<TPW * SL, 2 * TPW * NBN>::v];
 
-  // grid: x = patch (frame-major), y = row-pair blocks of all jobs (job-major).
-  // Consecutive workgroups are consecutive patches, so the round-robin
-  // placement of workgroups on the XCDs spreads every row block evenly.
-  const int by = blockIdx.y;
+  // 1-D grid over work units u = patch * nblk + row block (row blocks of all
+  // jobs, job-major).  Workgroup ids are dealt round-robin to the NXCD XCDs;
+  // XCD j runs the contiguous units [start_j, start_j + count_j), patch-major,
+  // so the overlapping patches of one image (which read the same pixels) run
+  // on one XCD and share its L2.  The map id -> u is a bijection whatever the
+  // hardware placement is.
+  const int nblk = jobs.blk_end[jobs.n - 1];
+  const int total = g.P * nblk;
+  const int xcd = (int)blockIdx.x % NXCD;
+  const int u = xcd * (total / NXCD) + min(xcd, total % NXCD) + (int)blockIdx.x / NXCD;
+  const int pu = u / nblk;
+  const int by = u - pu * nblk;
   int ji = 0;
   int blk0 = 0;
   if (jobs.n > 1 && by >= jobs.blk_end[0]) {
@@ -70,7 +79,7 @@ __global__ __launch_bounds__(WG) void k_rows_r2c(Geo g, RowJobs jobs, const floa
   const int NQ = (nr + 1) >> 1;
   const int q = q0 + tr;
   const bool active = q < NQ;
-  const int p = blockIdx.x;
+  const int p = pu;
   const int r0 = s.rlo + 2 * q;
   const int fr = p / g.Pf;
   const int k = p - fr * g.Pf;
@@ -172,7 +181,7 @@ static hipError_t rows_r2c_t(const Geo& g, const RowSrc* s, float2* const* out, 
   }
   if (blocks <= 0 || g.P <= 0) return hipSuccess;
   ProfScope ps(n == 1 ? "rows_r2c" : "rows_r2c_multi", st);
-  hipLaunchKernelGGL(k_rows_r2c<TW>, dim3((unsigned)g.P, (unsigned)blocks), dim3(WG), 0, st, g, jobs, tw);
+  hipLaunchKernelGGL(k_rows_r2c<TW>, dim3((unsigned)(g.P * blocks)), dim3(WG), 0, st, g, jobs, tw);
   return hipGetLastError();
 }
 
