@@ -121,6 +121,16 @@ int vmbd_ola_adjoint_xf(vmbd_plan* plan, int32_t nframes, const float* g, const 
 int vmbd_spectra(vmbd_plan* plan, int32_t nframes, int32_t kind, const float* z, void* spec,
                  vmbd_stream_t stream);
 
+/* The two OlaGPU constructors by name: vmbd_spectra with kind PSF ('F'
+ * instance, olaGPU.py:142-146) or IMAGE ('X' instance, olaGPU.py:148-157). */
+int vmbd_cache_psf_spectra(vmbd_plan* plan, int32_t nframes, const float* f, void* spec, vmbd_stream_t stream);
+int vmbd_cache_image_spectra(vmbd_plan* plan, int32_t nframes, const float* x, void* spec, vmbd_stream_t stream);
+
+/* The normalised per-patch windows, ws (csf0*csf1) x sw0 x sw1 float32 on the
+ * device (win2winaux.ws_gpu, imagetools.py:439-454), expanded from the plan's
+ * window classes. */
+int vmbd_windows(const vmbd_plan* plan, float* ws, vmbd_stream_t stream);
+
 /* OlaGPU.cnv with the cached operand (olaGPU.py:170-257):
  * kind PSF: u = image x; kind IMAGE: u = PSF stack.  y: sy0 x sy1. */
 int vmbd_ola_forward_cached(vmbd_plan* plan, int32_t nframes, int32_t kind, const void* spec, const float* u,

@@ -53,6 +53,13 @@ int vmbd_pbb_stats(const float* x, const float* oldx, const float* g, const floa
 int vmbd_pbb_update(float* xnew, const float* x, const float* g, int64_t n, const double* partials, int32_t iter,
                     int32_t unconstrained, float* step_out, vmbd_stream_t stream);
 
+/* One projected BB iteration in one call (vmbd_pbb_stats + vmbd_pbb_update,
+ * optGPU.py:200-245): xnew = P(x - step g) with the step of iteration `iter`
+ * computed on the device from (x - oldx, g - oldg); partials:
+ * 3 * vmbd_reduce_blocks(n) doubles. */
+int vmbd_pbb_step(float* xnew, const float* x, const float* oldx, const float* g, const float* oldg, int64_t n,
+                  int32_t iter, int32_t unconstrained, double* partials, float* step_out, vmbd_stream_t stream);
+
 /* Start pair of PBB (optGPU.py:256-297, replaces the host comparison of
  * the two start objectives): dots = device [r.r, z.z] at the first point and
  * at the second, objective r.r/2 + beta z.z/2; when the second point is not

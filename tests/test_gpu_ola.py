@@ -221,3 +221,40 @@ def test_graph_capture_replay(cuda):
         torch.cuda.synchronize()
         assert torch.equal(y, e[0]) and torch.equal(xb, e[1]) and torch.equal(fb, e[2]), it
         _ = (xt * 2).sum().item()  # unrelated eager work between replays
+
+
+def test_named_entry_points(cuda):
+    """vmbd_windows expands the window classes to win2winaux's ws exactly;
+    vmbd_cache_{psf,image}_spectra equal vmbd_spectra; vmbd_pbb_step equals
+    vmbd_pbb_stats + vmbd_pbb_update."""
+    import torch
+    from vmbd_amd import _lib
+    from vmbd_amd.imagetools import win2winaux
+    from vmbd_amd.ola import OlaPlan
+    print('case named entry points', flush=True)
+    sx, csf, sf = (120, 100), (3, 4), (6, 8)
+    wa = win2winaux(sx, csf, 0.5)
+    plan = OlaPlan(sx, sf, wa, 'valid')
+    assert torch.equal(plan.windows().cpu(), torch.as_tensor(wa.ws))
+    x, f, _ = case_inputs(80, sx, csf, sf)
+    xt = torch.as_tensor(x.astype(np.float32), device=cuda)
+    ft = torch.as_tensor(f.astype(np.float32), device=cuda)
+    for kind, name, z in (('F', 'vmbd_cache_psf_spectra', ft), ('X', 'vmbd_cache_image_spectra', xt)):
+        ref = plan.spectra(kind, z)
+        got = torch.empty_like(ref)
+        _lib.call(name, plan._h, 1, _lib.ptr(z), _lib.ptr(got), _lib.stream_ptr())
+        assert torch.equal(got, ref), name
+    rng = np.random.default_rng(81)
+    n = 7777
+    a, b, c, d = (torch.as_tensor(rng.standard_normal(n).astype(np.float32), device=cuda) for _ in range(4))
+    part = torch.empty(3 * _lib.lib().vmbd_reduce_blocks(n), dtype=torch.float64, device=cuda)
+    s1, s2 = torch.empty(1, device=cuda), torch.empty(1, device=cuda)
+    o1, o2 = torch.empty_like(a), torch.empty_like(a)
+    for it in (1, 2):
+        _lib.call('vmbd_pbb_stats', _lib.ptr(a), _lib.ptr(b), _lib.ptr(c), _lib.ptr(d), n, 0, _lib.ptr(part), None,
+                  _lib.stream_ptr())
+        _lib.call('vmbd_pbb_update', _lib.ptr(o1), _lib.ptr(a), _lib.ptr(c), n, _lib.ptr(part), it, 0, _lib.ptr(s1),
+                  _lib.stream_ptr())
+        _lib.call('vmbd_pbb_step', _lib.ptr(o2), _lib.ptr(a), _lib.ptr(b), _lib.ptr(c), _lib.ptr(d), n, it, 0,
+                  _lib.ptr(part), _lib.ptr(s2), _lib.stream_ptr())
+        assert torch.equal(o1, o2) and torch.equal(s1, s2), it

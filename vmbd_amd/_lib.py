@@ -47,6 +47,9 @@ SIGNATURES = {
     'vmbd_ola_adjoint_f': (ctypes.c_int, [_P, _I, _F, _F, _F, _I, _P]),
     'vmbd_ola_adjoint_xf': (ctypes.c_int, [_P, _I, _F, _F, _F, _F, _F, _I, _I, _P]),
     'vmbd_spectra': (ctypes.c_int, [_P, _I, _I, _F, _P, _P]),
+    'vmbd_cache_psf_spectra': (ctypes.c_int, [_P, _I, _F, _P, _P]),
+    'vmbd_cache_image_spectra': (ctypes.c_int, [_P, _I, _F, _P, _P]),
+    'vmbd_windows': (ctypes.c_int, [_P, _F, _P]),
     'vmbd_ola_forward_cached': (ctypes.c_int, [_P, _I, _I, _P, _F, _F, _P]),
     'vmbd_ola_adjoint_cached': (ctypes.c_int, [_P, _I, _I, _P, _F, _F, _I, _P]),
     # plain (reference-decomposition) kernels, include/vmbd_plain.h
@@ -75,6 +78,7 @@ SIGNATURES = {
     'vmbd_axpby': (ctypes.c_int, [_F, _F, ctypes.c_float, _F, ctypes.c_float, ctypes.c_int64, _P]),
     'vmbd_mul': (ctypes.c_int, [_F, _F, _F, ctypes.c_int64, _P]),
     'vmbd_ew': (ctypes.c_int, [_F, _F, _F, ctypes.c_float, ctypes.c_int64, _I, _P]),
+    'vmbd_pbb_step': (ctypes.c_int, [_F, _F, _F, _F, _F, ctypes.c_int64, _I, _I, _P, _F, _P]),
     'vmbd_pbb_select': (ctypes.c_int, [_F, _F, _F, _F, ctypes.c_int64, _P, ctypes.c_double, _P]),
     'vmbd_tv': (ctypes.c_int, [_F, _I, _I, _I, ctypes.c_float, _P, _P, _P]),
     'vmbd_tv_grad': (ctypes.c_int, [_F, _F, _I, _I, _I, ctypes.c_float, _P, ctypes.c_float, _I, _P]),

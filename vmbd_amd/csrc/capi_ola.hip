@@ -1,6 +1,8 @@
 // C ABI of the EFF/OLA operator: plan object + operator entry points.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -21,11 +23,11 @@ struct vmbd_plan {
   float* win = nullptr;
   int* win_idx = nullptr;
   int n_win = 0;
-  float2* wsG = nullptr;  // [P][NB][ext0]
-  float2* wsX = nullptr;  // [P][NB][sw0]
-  float2* wsF = nullptr;  // [P][NB][sf0]
-  float2* Y1 = nullptr;   // [P][ext0][NB]
-  float2* Y2 = nullptr;   // [P][sf0][NB]
+  float2* wsG = nullptr;  // [P][NB][row_pitch(ext0)]
+  float2* wsX = nullptr;  // [P][NB][row_pitch(sw0)]
+  float2* wsF = nullptr;  // [P][NB][row_pitch(sf0)]
+  float2* Y1 = nullptr;   // [P][ext0][NBP]
+  float2* Y2 = nullptr;   // [P][sf0][NBP]
   size_t bytes = 0;
   int mc_fwd = 0, mc_adj = 0;
   int max_frames = 1;
@@ -140,8 +142,10 @@ RowSrc src_residual(const vmbd_plan* p, const float* gres, int exact) {
   return s;
 }
 
-ColIn raw_in(const float2* ws, const RowSrc& s) { return ColIn{ws, s.rlo, s.rhi - s.rlo, 0}; }
-ColIn spec_in(const void* spec, int Th) { return ColIn{(const float2*)spec, 0, Th, 1}; }
+ColIn raw_in(const float2* ws, const RowSrc& s) {
+  return ColIn{ws, s.rlo, s.rhi - s.rlo, 0, row_pitch(s.rhi - s.rlo)};
+}
+ColIn spec_in(const void* spec, int Th) { return ColIn{(const float2*)spec, 0, Th, 1, Th}; }
 
 float inv_scale(const Geo& g) { return 1.0f / (float)((double)g.Th * (double)g.Tw); }
 
@@ -246,6 +250,7 @@ vmbd_plan* vmbd_plan_create(const vmbd_geom* gm, const float* win_table, int32_t
   g.Th = G.Th;
   g.Tw = G.Tw;
   g.NB = G.Tw / 2 + 1;
+  g.NBP = (g.NB + 15) & ~15;
   g.ext0 = G.sw0 + G.sfh - 1;
   g.ext1 = G.sw1 + G.sfw - 1;
   switch (G.mode) {
@@ -317,11 +322,11 @@ vmbd_plan* vmbd_plan_create(const vmbd_geom* gm, const float* win_table, int32_t
     }
   }
   size_t tot = 0;
-  const size_t P = (size_t)g.Pf * p->max_frames, NB = g.NB;
+  const size_t P = (size_t)g.Pf * p->max_frames, NB = g.NB, NBP = g.NBP;
   bool ok = dev_alloc(&p->tw, TW_LEN, tot) && dev_alloc(&p->win, (size_t)n_win * g.sw0 * g.sw1, tot) &&
-            dev_alloc(&p->win_idx, g.Pf, tot) && dev_alloc(&p->wsG, P * NB * g.ext0, tot) &&
-            dev_alloc(&p->wsX, P * NB * g.sw0, tot) && dev_alloc(&p->wsF, P * NB * g.sf0, tot) &&
-            dev_alloc(&p->Y1, P * NB * g.ext0, tot) && dev_alloc(&p->Y2, P * NB * g.sf0, tot);
+            dev_alloc(&p->win_idx, g.Pf, tot) && dev_alloc(&p->wsG, P * NB * row_pitch(g.ext0), tot) &&
+            dev_alloc(&p->wsX, P * NB * row_pitch(g.sw0), tot) && dev_alloc(&p->wsF, P * NB * row_pitch(g.sf0), tot) &&
+            dev_alloc(&p->Y1, P * NBP * g.ext0, tot) && dev_alloc(&p->Y2, P * NBP * g.sf0, tot);
   if (!ok) {
     plan_free(p);
     vmbd_fail(VMBD_E_NOMEM, "vmbd_plan_create: hipMalloc failed");
@@ -447,6 +452,38 @@ int vmbd_ola_adjoint_xf(vmbd_plan* p, int32_t nframes, const float* gres, const 
   VMBD_LAUNCH(launch_rows_c2r_ola(g, dst_adjoint_x(p, xbar), p->tw, st));
   VMBD_LAUNCH(launch_rows_c2r_stack(g, p->Y2, g.sf0, fbar, g.sf0, g.sf1, p->tw, st));
   return VMBD_OK;
+}
+
+namespace {
+// per-patch windows ws[k] = table[idx[k]] (the plan keeps the class table)
+__global__ __launch_bounds__(256) void k_windows(float* ws, const float* table, const int* idx, int P, int n) {
+  const long long total = (long long)P * n;
+  for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long long)gridDim.x * 256) {
+    const long long k = e / n;
+    ws[e] = table[(long long)idx[k] * n + (e - k * n)];
+  }
+}
+}  // namespace
+
+int vmbd_windows(const vmbd_plan* p, float* ws, vmbd_stream_t stream) {
+  vmbd_clear_error();
+  if (!p || !ws) return vmbd_fail(VMBD_E_ARG, "vmbd_windows: null pointer");
+  const Geo& g = p->g;
+  const int n = g.sw0 * g.sw1;
+  const long long total = (long long)g.Pf * n;
+  const unsigned blocks = (unsigned)std::min<long long>((total + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_windows, dim3(blocks), dim3(256), 0, (hipStream_t)stream, ws, p->win, p->win_idx, g.Pf, n);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return vmbd_fail_hip("vmbd_windows", e);
+  return VMBD_OK;
+}
+
+int vmbd_cache_psf_spectra(vmbd_plan* p, int32_t nframes, const float* f, void* spec, vmbd_stream_t stream) {
+  return vmbd_spectra(p, nframes, VMBD_CACHE_PSF, f, spec, stream);
+}
+
+int vmbd_cache_image_spectra(vmbd_plan* p, int32_t nframes, const float* x, void* spec, vmbd_stream_t stream) {
+  return vmbd_spectra(p, nframes, VMBD_CACHE_IMAGE, x, spec, stream);
 }
 
 int vmbd_spectra(vmbd_plan* p, int32_t nframes, int32_t kind, const float* z, void* spec, vmbd_stream_t stream) {

@@ -19,6 +19,7 @@ struct Geo {
   int sw0, sw1;
   int hop0, hop1;
   int Th, Tw, NB;
+  int NBP;  // row pitch (bins) of the column-pass outputs: NB rounded up to 16 (128-byte lines)
   int ext0, ext1;
   int oc0, oc1;
   int sy0, sy1;
@@ -50,15 +51,20 @@ struct RowJobs {
   int n;
 };
 
-// Column-pass input: either raw row-transform output laid out [P][NB][nr]
-// holding tile rows [rlo, rlo+nr) (spec == 0), or a cached spectrum laid out
-// [P][NB][Th] (spec == 1).
+// Row-pass output: [P][NB][row_pitch(nr)] holding tile rows [rlo, rlo+nr);
+// the pitch rounds nr up to 16 rows so every bin column starts on a 128-byte
+// line and a workgroup's 32 rows fill whole lines.
+__host__ __device__ inline int row_pitch(int nr) { return (nr + 15) & ~15; }
+
+// Column-pass input: either raw row-transform output (spec == 0, pitch =
+// row_pitch(nr)), or a cached spectrum laid out [P][NB][Th] (spec == 1,
+// pitch = Th).
 struct ColIn {
   const float2* p;
-  int rlo, nr, spec;
+  int rlo, nr, spec, pitch;
 };
 
-// Column-pass output rows [0, nrows) in layout [P][nrows][NB].
+// Column-pass output rows [0, nrows) in layout [P][nrows][NBP].
 struct ColOut {
   float2* p;
   int nrows;
@@ -67,7 +73,7 @@ struct ColOut {
 // Inverse row transform + overlap-add gather + crop into an output image.
 // Output pixel (i, j) sums, over every patch k whose local extent covers the
 // OLA coordinate (i+oc0, j+oc1), the spatial value at local (r, c) =
-// (i+oc0-m*hop0, j+oc1-n*hop1) taken from src rows [P][srow][NB]; the value is
+// (i+oc0-m*hop0, j+oc1-n*hop1) taken from src rows [P][srow][NBP]; the value is
 // multiplied by win[win_idx[k]][r][c] when win != null.
 struct OlaDst {
   float* out;

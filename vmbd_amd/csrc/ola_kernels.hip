@@ -6,12 +6,14 @@
 //
 //   rows_r2c      chop + window + zero pad fused into the load, real row FFTs
 //                 (two rows packed in one complex transform), half spectrum
-//                 written bin-major [P][NB][rows] so the column pass reads
-//                 contiguous runs.  Rows that are all zero are never computed.
+//                 written bin-major [P][NB][row_pitch(rows)] so the column
+//                 pass reads contiguous runs.  Rows that are all zero are
+//                 never computed.
 //   cols          per (patch, strip of bins): column FFTs of 1..3 inputs, the
 //                 spectrum product (conj for the transposes), inverse column
 //                 FFT; only the output rows that are needed are stored,
-//                 row-major [P][rows][NB].
+//                 row-major [P][rows][NBP] (NBP = NB rounded up to 16 bins,
+//                 so a 16-bin strip of a row is one 128-byte line).
 //   rows_c2r_ola  one workgroup per output row: inverse real row FFTs of every
 //                 patch row that lands on it (two patches per transform), then
 //                 a fixed-order overlap-add gather and crop (deterministic,
@@ -150,17 +152,18 @@ __global__ __launch_bounds__(WG) void k_rows_r2c(Geo g, RowJobs jobs, const floa
   __syncthreads();
   // bin-major store: consecutive lanes -> consecutive rows of one bin; the
   // block's rows are the contiguous range [2*q0, 2*q0 + 2*TPW) of patch p
-  float2* dst = out + (size_t)p * NBN * nr + 2 * q0;
+  const int nrp = row_pitch(nr);
+  float2* dst = out + (size_t)p * NBN * nrp + 2 * q0;
   const int nrow = min(2 * TPW, nr - 2 * q0);
   if constexpr (2 * TPW <= WG) {
     const int rl = tid & (2 * TPW - 1);
     if (rl < nrow)
-      for (int c = tid / (2 * TPW); c < NBN; c += WG / (2 * TPW)) dst[(size_t)c * nr + rl] = sm[rl * NBN + c];
+      for (int c = tid / (2 * TPW); c < NBN; c += WG / (2 * TPW)) dst[(size_t)c * nrp + rl] = sm[rl * NBN + c];
   } else {
     for (int e = tid; e < 2 * TPW * NBN; e += WG) {
       const int c = e / (2 * TPW);
       const int rl = e - c * (2 * TPW);
-      if (rl < nrow) dst[(size_t)c * nr + rl] = sm[rl * NBN + c];
+      if (rl < nrow) dst[(size_t)c * nrp + rl] = sm[rl * NBN + c];
     }
   }
 }
@@ -214,7 +217,7 @@ __device__ __forceinline__ void col_load(float2 (&v)[16], const ColIn& in, int p
 #pragma unroll
     for (int r = 0; r < 16; ++r) v[r] = active ? col[t + TPT * r] : make_float2(0.f, 0.f);
   } else {
-    const float2* col = in.p + ((size_t)p * NB + c) * in.nr;
+    const float2* col = in.p + ((size_t)p * NB + c) * in.pitch;
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
       const int rr = t + TPT * r - in.rlo;
@@ -223,11 +226,11 @@ __device__ __forceinline__ void col_load(float2 (&v)[16], const ColIn& in, int p
   }
 }
 
-// stage rows [0,nrows) of the strip in LDS and store row-major [P][nrows][NB].
+// stage rows [0,nrows) of the strip in LDS and store row-major [P][nrows][NBP].
 // The staging area aliases the FFT slices: barriers on both sides.
 template <int TH, int CS>
 __device__ __forceinline__ void col_store_rows(const float2 (&v)[16], float2* st, const ColOut& o, int p, int strip,
-                                               int NB, int cc, int t) {
+                                               int NB, int NBP, int cc, int t) {
   constexpr int TPT = TH / 16;
   __syncthreads();  // all transforms of the workgroup are done with the slices
 #pragma unroll
@@ -239,8 +242,8 @@ __device__ __forceinline__ void col_store_rows(const float2 (&v)[16], float2* st
   const int c0 = strip * CS;
   const int ci = threadIdx.x & (CS - 1);
   if (c0 + ci < NB) {
-    float2* dst = o.p + (size_t)p * o.nrows * NB + c0 + ci;
-    for (int r = threadIdx.x / CS; r < o.nrows; r += WG / CS) dst[(size_t)r * NB] = st[r * (CS + 1) + ci];
+    float2* dst = o.p + (size_t)p * o.nrows * NBP + c0 + ci;
+    for (int r = threadIdx.x / CS; r < o.nrows; r += WG / CS) dst[(size_t)r * NBP] = st[r * (CS + 1) + ci];
   }
   __syncthreads();  // staging drained before the slices are reused
 }
@@ -286,14 +289,14 @@ __global__ __launch_bounds__(WG) void k_cols(Geo g, ColIn a, ColIn b, ColIn c3, 
     for (int m = 0; m < 16; ++m)
       B[m] = cscale((OP == COL_FWD) ? cmul(A[m], B[m]) : cmulc(A[m], B[m]), scale);
     if (wact) fft<TH, +1>(B, sl, t, T, tw);
-    col_store_rows<TH, CS>(B, sm, o1, p, strip, NB, cc, t);
+    col_store_rows<TH, CS>(B, sm, o1, p, strip, NB, g.NBP, cc, t);
     if constexpr (OP == COL_ADJ2) {
       col_load<TH>(B, c3, p, c, NB, t, active);
       if (!c3.spec && wact) fft<TH, -1>(B, sl, t, T, tw);
 #pragma unroll
       for (int m = 0; m < 16; ++m) B[m] = cscale(cmulc(A[m], B[m]), scale);
       if (wact) fft<TH, +1>(B, sl, t, T, tw);
-      col_store_rows<TH, CS>(B, sm, o2, p, strip, NB, cc, t);
+      col_store_rows<TH, CS>(B, sm, o2, p, strip, NB, g.NBP, cc, t);
     }
   }
 }
@@ -414,8 +417,8 @@ __global__ __launch_bounds__(WG) void k_rows_c2r_ola(Geo g, OlaDst d, const floa
     const int ka = kf + m * g.csf1 + na;
     const bool hasb = nb < g.csf1;
     const bool rok = act && r >= 0 && r < d.srow;
-    const float2* ra = rok ? d.src + ((size_t)ka * d.srow + r) * g.NB : d.src;
-    const float2* rb = (rok && hasb) ? ra + (size_t)d.srow * g.NB : ra;
+    const float2* ra = rok ? d.src + ((size_t)ka * d.srow + r) * g.NBP : d.src;
+    const float2* rb = (rok && hasb) ? ra + (size_t)d.srow * g.NBP : ra;
     // window rows of both patches (addresses resolved before the transform)
     const bool wrow = d.win && rok && r < g.sw0;
     const float* wa = wrow ? d.win + ((size_t)d.win_idx[ka - kf] * g.sw0 + r) * g.sw1 : nullptr;
@@ -533,9 +536,9 @@ __global__ __launch_bounds__(WG) void k_rows_c2r_stack(Geo g, const float2* __re
   if (blockIdx.y * TPW + (int)(threadIdx.x >> 6) * (64 / TPT) >= NQ) return;
   const int p = blockIdx.x;
   const int r0 = 2 * q, r1 = 2 * q + 1;
-  const float2* ra = act ? src + ((size_t)p * srow + r0) * g.NB : src;
+  const float2* ra = act ? src + ((size_t)p * srow + r0) * g.NBP : src;
   const bool hasb = act && r1 < nrows;
-  const float2* rb = hasb ? ra + g.NB : ra;
+  const float2* rb = hasb ? ra + g.NBP : ra;
   Twiddles<TW> T;
   twiddles_init<TW>(T, t, tw);
   float2 v[16];

@@ -389,6 +389,20 @@ int vmbd_pbb_update(float* xnew, const float* x, const float* g, int64_t n, cons
   SOLVER_DONE("vmbd_pbb_update");
 }
 
+int vmbd_pbb_step(float* xnew, const float* x, const float* oldx, const float* g, const float* oldg, int64_t n,
+                  int32_t iter, int32_t unconstrained, double* partials, float* step_out, vmbd_stream_t stream) {
+  vmbd_clear_error();
+  if (!xnew || !x || !oldx || !g || !oldg || !partials || n <= 0)
+    return vmbd_fail(VMBD_E_ARG, "vmbd_pbb_step: bad argument");
+  hipStream_t st = (hipStream_t)stream;
+  const int nb = grid_for(n);
+  ProfScope ps("pbb_step", st);
+  hipLaunchKernelGGL(k_pbb_stats, dim3(nb), dim3(TB), 0, st, x, oldx, g, oldg, (long long)n, unconstrained, partials);
+  hipLaunchKernelGGL(k_pbb_update, dim3(nb), dim3(TB), 0, st, xnew, x, g, (long long)n, partials, nb, iter,
+                     unconstrained, step_out);
+  SOLVER_DONE("vmbd_pbb_step");
+}
+
 int vmbd_pbb_select(float* x, float* oldx, float* g, float* oldg, int64_t n, const double* dots, double beta,
                     vmbd_stream_t stream) {
   vmbd_clear_error();

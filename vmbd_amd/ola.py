@@ -196,6 +196,12 @@ class OlaPlan:
         self._rows_key = self._key(x, f, nf)
         return xb, fb
 
+    def windows(self):
+        """The (P, sw0, sw1) normalised windows as the kernels apply them."""
+        ws = torch.empty((self.P,) + self.sw, dtype=torch.float32, device=self.device)
+        _lib.check(_lib.lib().vmbd_windows(self._h, _lib.ptr(ws), _lib.stream_ptr()), 'windows')
+        return ws
+
     # -- cached spectra (OlaGPU semantics) ----------------------------------
     def spectra(self, kind, z, out=None):
         """kind 'F': z = PSF stack; kind 'X': z = image.  Returns a complex64
