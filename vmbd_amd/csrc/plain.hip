@@ -397,16 +397,29 @@ int vmbd_fft2(void* data, int32_t batch, int32_t Th, int32_t Tw, int32_t dir, fl
   return VMBD_OK;
 }
 
+namespace {
+// exp(-2 pi i m / TW_LEN) in FP64, rounded once; built on first use and never
+// written again, so asynchronous copies from it need no wait
+struct TwiddleTable {
+  float2 v[TW_LEN];
+  TwiddleTable() {
+    for (int m = 0; m < TW_LEN; ++m) {
+      const double a = -2.0 * 3.14159265358979323846 * (double)m / (double)TW_LEN;
+      v[m] = make_float2((float)cos(a), (float)sin(a));
+    }
+  }
+};
+const TwiddleTable& twiddle_table() {
+  static const TwiddleTable t;
+  return t;
+}
+}  // namespace
+
 int vmbd_twiddles(void* tw, vmbd_stream_t stream) {
   vmbd_clear_error();
   if (!tw) return vmbd_fail(VMBD_E_ARG, "vmbd_twiddles: null");
-  static float2 host[TW_LEN];
-  for (int m = 0; m < TW_LEN; ++m) {
-    const double a = -2.0 * 3.14159265358979323846 * (double)m / (double)TW_LEN;
-    host[m] = make_float2((float)cos(a), (float)sin(a));
-  }
-  hipError_t e = hipMemcpyAsync(tw, host, sizeof(host), hipMemcpyHostToDevice, (hipStream_t)stream);
-  if (e == hipSuccess) e = hipStreamSynchronize((hipStream_t)stream);
+  const TwiddleTable& t = twiddle_table();
+  hipError_t e = hipMemcpyAsync(tw, t.v, sizeof(t.v), hipMemcpyHostToDevice, (hipStream_t)stream);
   if (e != hipSuccess) return vmbd_fail_hip("vmbd_twiddles", e);
   return VMBD_OK;
 }
