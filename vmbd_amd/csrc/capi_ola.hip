@@ -22,6 +22,9 @@ struct vmbd_plan {
   float2* tw = nullptr;
   float* win = nullptr;
   int* win_idx = nullptr;
+  int* win_slot = nullptr;   // [Pf] slot of patch k among the distinct windows of its patch row
+  int* row_class = nullptr;  // [csf0][nslot] window class of each slot (-1 unused)
+  int nslot = 0;
   int n_win = 0;
   float2* wsG = nullptr;  // [P][NB][row_pitch(ext0)]
   float2* wsX = nullptr;  // [P][NB][row_pitch(sw0)]
@@ -64,6 +67,8 @@ void plan_free(vmbd_plan* p) {
   (void)hipFree(p->tw);
   (void)hipFree(p->win);
   (void)hipFree(p->win_idx);
+  (void)hipFree(p->win_slot);
+  (void)hipFree(p->row_class);
   (void)hipFree(p->wsG);
   (void)hipFree(p->wsX);
   (void)hipFree(p->wsF);
@@ -182,6 +187,9 @@ OlaDst dst_adjoint_x(const vmbd_plan* p, float* xbar) {
   d.srow = g.sw0;
   d.win = p->win;
   d.win_idx = p->win_idx;
+  d.win_slot = p->win_slot;
+  d.row_class = p->row_class;
+  d.nslot = p->nslot;
   d.mc = p->mc_adj;
   d.alias = ola_single_round(g, d);
   return d;
@@ -308,6 +316,23 @@ vmbd_plan* vmbd_plan_create(const vmbd_geom* gm, const float* win_table, int32_t
   p->n_win = n_win;
   p->max_frames = G.max_frames > 0 ? G.max_frames : 1;
   (void)hipGetDevice(&p->device);
+  // distinct windows per patch row (the x-adjoint stages one row of each)
+  std::vector<int> slot(g.Pf), rclass;
+  {
+    std::vector<std::vector<int>> rows(g.csf0);
+    for (int m = 0; m < g.csf0; ++m)
+      for (int n = 0; n < g.csf1; ++n) {
+        const int c = win_idx[m * g.csf1 + n];
+        auto& v = rows[m];
+        int sidx = (int)(std::find(v.begin(), v.end(), c) - v.begin());
+        if (sidx == (int)v.size()) v.push_back(c);
+        slot[m * g.csf1 + n] = sidx;
+      }
+    for (auto& v : rows) p->nslot = std::max(p->nslot, (int)v.size());
+    rclass.assign((size_t)g.csf0 * p->nslot, -1);
+    for (int m = 0; m < g.csf0; ++m)
+      for (size_t j = 0; j < rows[m].size(); ++j) rclass[(size_t)m * p->nslot + j] = rows[m][j];
+  }
   // rows of the OLA gather buffer
   p->mc_fwd = max_cover(g.csf0, g.hop0, g.ext0, g.oc0, g.oc0 + g.sy0);
   p->mc_adj = max_cover(g.csf0, g.hop0, g.sw0, 0, g.H);
@@ -326,7 +351,8 @@ vmbd_plan* vmbd_plan_create(const vmbd_geom* gm, const float* win_table, int32_t
   bool ok = dev_alloc(&p->tw, TW_LEN, tot) && dev_alloc(&p->win, (size_t)n_win * g.sw0 * g.sw1, tot) &&
             dev_alloc(&p->win_idx, g.Pf, tot) && dev_alloc(&p->wsG, P * NB * row_pitch(g.ext0), tot) &&
             dev_alloc(&p->wsX, P * NB * row_pitch(g.sw0), tot) && dev_alloc(&p->wsF, P * NB * row_pitch(g.sf0), tot) &&
-            dev_alloc(&p->Y1, P * NBP * g.ext0, tot) && dev_alloc(&p->Y2, P * NBP * g.sf0, tot);
+            dev_alloc(&p->Y1, P * NBP * g.ext0, tot) && dev_alloc(&p->Y2, P * NBP * g.sf0, tot) &&
+            dev_alloc(&p->win_slot, g.Pf, tot) && dev_alloc(&p->row_class, rclass.size(), tot);
   if (!ok) {
     plan_free(p);
     vmbd_fail(VMBD_E_NOMEM, "vmbd_plan_create: hipMalloc failed");
@@ -342,6 +368,9 @@ vmbd_plan* vmbd_plan_create(const vmbd_geom* gm, const float* win_table, int32_t
   if (e == hipSuccess)
     e = hipMemcpy(p->win, win_table, (size_t)n_win * g.sw0 * g.sw1 * sizeof(float), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(p->win_idx, win_idx, (size_t)g.Pf * sizeof(int), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(p->win_slot, slot.data(), slot.size() * sizeof(int), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(p->row_class, rclass.data(), rclass.size() * sizeof(int), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
     plan_free(p);
     vmbd_fail_hip("vmbd_plan_create: hipMemcpy", e);

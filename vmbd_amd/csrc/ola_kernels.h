@@ -84,6 +84,12 @@ struct OlaDst {
   int srow;
   const float* win;
   const int* win_idx;
+  // windows staged in LDS per output row: patch k = m*csf1 + n uses slot
+  // win_slot[k] of its patch row, slot s of patch row m holds class
+  // row_class[m*nslot + s] (-1: unused); nslot distinct windows per patch row
+  const int* win_slot;
+  const int* row_class;
+  int nslot;
   int mc;     // max number of patch rows covering one output row
   int alias;  // 1: all transforms of a row fit one round; gather buffer aliases the FFT slices
 };

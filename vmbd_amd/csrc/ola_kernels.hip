@@ -48,18 +48,23 @@ __global__ __launch_bounds__(WG) void k_rows_r2c(Geo g, RowJobs jobs, const floa
   constexpr int SL = FftLds<TW>::floats2;
   __shared__ float2 sm[cmax<TPW * SL, 2 * TPW * NBN>::v];
 
-  // 1-D grid over work units u = patch * nblk + row block (row blocks of all
-  // jobs, job-major).  Workgroup ids are dealt round-robin to the NXCD XCDs;
-  // XCD j runs the contiguous units [start_j, start_j + count_j), patch-major,
-  // so the overlapping patches of one image (which read the same pixels) run
-  // on one XCD and share its L2.  The map id -> u is a bijection whatever the
-  // hardware placement is.
+  // 1-D grid over work units (patch, row block), row blocks of all jobs
+  // job-major.  Workgroup ids are dealt round-robin to the NXCD XCDs; XCD j
+  // runs the contiguous units [start_j, start_j + count_j).  Units are ordered
+  // in tiles of TP = ceil(P / NXCD) consecutive patches (one XCD's share),
+  // row-block-major inside a tile, so neighbouring patches, which read the
+  // same pixels, run on one XCD at the same time and share its L2.  The map
+  // id -> (patch, row block) is a bijection whatever the hardware placement.
   const int nblk = jobs.blk_end[jobs.n - 1];
   const int total = g.P * nblk;
   const int xcd = (int)blockIdx.x % NXCD;
   const int u = xcd * (total / NXCD) + min(xcd, total % NXCD) + (int)blockIdx.x / NXCD;
-  const int pu = u / nblk;
-  const int by = u - pu * nblk;
+  const int TP = (g.P + NXCD - 1) / NXCD;
+  const int tile = u / (TP * nblk);
+  const int ur = u - tile * TP * nblk;
+  const int Tt = min(TP, g.P - tile * TP);
+  const int by = ur / Tt;
+  const int pu = tile * TP + (ur - by * Tt);
   int ji = 0;
   int blk0 = 0;
   if (jobs.n > 1 && by >= jobs.blk_end[0]) {
@@ -375,6 +380,8 @@ __device__ __forceinline__ void c2r_load(float2 (&v)[16], const float2* ra, cons
 // ----------------------------------------------------------------------------
 // rows_c2r_ola: one workgroup per output row
 // ----------------------------------------------------------------------------
+__host__ __device__ int ola_window_offset_floats(const Geo& g, const OlaDst& d);
+
 template <int TW>
 __global__ __launch_bounds__(WG) void k_rows_c2r_ola(Geo g, OlaDst d, const float2* __restrict__ tw) {
   constexpr int TPT = TW / 16;
@@ -406,6 +413,24 @@ __global__ __launch_bounds__(WG) void k_rows_c2r_ola(Geo g, OlaDst d, const floa
   Twiddles<TW> T;
   twiddles_init<TW>(T, t, tw);
 
+  // x-adjoint: the window rows this output row needs, one per (covering patch
+  // row, window class of that row), staged in LDS past the gather buffer
+  float* wl = (float*)(dyn) + ola_window_offset_floats(g, d);
+  if (d.win) {
+    const int npair = Mc * d.nslot;
+    for (int pr = (int)(threadIdx.x >> 6); pr < npair; pr += WG / 64) {
+      const int mi = pr / d.nslot;
+      const int sl = pr - mi * d.nslot;
+      const int m = mlo + mi;
+      const int r = u - m * g.hop0;
+      const int cls = d.row_class[m * d.nslot + sl];
+      const bool ok = cls >= 0 && r >= 0 && r < g.sw0;
+      const float* src = ok ? d.win + ((size_t)cls * g.sw0 + r) * g.sw1 : d.win;
+      for (int c = threadIdx.x & 63; c < g.sw1; c += 64) wl[pr * g.sw1 + c] = ok ? src[c] : 0.f;
+    }
+    __syncthreads();
+  }
+
   for (int base = 0; base < NT; base += TPW) {
     const int tix = base + tr;
     const bool act = tix < NT;
@@ -419,10 +444,10 @@ __global__ __launch_bounds__(WG) void k_rows_c2r_ola(Geo g, OlaDst d, const floa
     const bool rok = act && r >= 0 && r < d.srow;
     const float2* ra = rok ? d.src + ((size_t)ka * d.srow + r) * g.NBP : d.src;
     const float2* rb = (rok && hasb) ? ra + (size_t)d.srow * g.NBP : ra;
-    // window rows of both patches (addresses resolved before the transform)
+    // staged window rows of both patches
     const bool wrow = d.win && rok && r < g.sw0;
-    const float* wa = wrow ? d.win + ((size_t)d.win_idx[ka - kf] * g.sw0 + r) * g.sw1 : nullptr;
-    const float* wbp = (wrow && hasb) ? d.win + ((size_t)d.win_idx[ka + 1 - kf] * g.sw0 + r) * g.sw1 : wa;
+    const float* wa = wrow ? wl + (mi * d.nslot + d.win_slot[ka - kf]) * g.sw1 : wl;
+    const float* wbp = (wrow && hasb) ? wl + (mi * d.nslot + d.win_slot[ka + 1 - kf]) * g.sw1 : wa;
     float2 v[16];
     // a wavefront with no item in this round skips its transforms
     const bool wact = base + (int)(threadIdx.x >> 6) * (64 / TPT) < NT;
@@ -477,16 +502,23 @@ __global__ __launch_bounds__(WG) void k_rows_c2r_ola(Geo g, OlaDst d, const floa
   }
 }
 
+// float offset of the staged windows: past the FFT slices and the gather
+// buffer (which alias each other in single-round launches)
+__host__ __device__ int ola_window_offset_floats(const Geo& g, const OlaDst& d) {
+  const int TPT = g.Tw / 16;
+  const int TPW = WG / TPT;
+  const int fl = TPW * (g.Tw + g.Tw / 16) * 2;
+  const int rb = d.mc * g.csf1 * d.ext1;
+  return d.alias ? (fl > rb ? fl : rb) : fl + rb;
+}
+
 size_t ola_rowbuf_bytes(const Geo& g, const OlaDst& d) {
   return (size_t)d.mc * g.csf1 * d.ext1 * sizeof(float);
 }
 
 size_t ola_shared_bytes(const Geo& g, const OlaDst& d) {
-  const int TPT = g.Tw / 16;
-  const int TPW = WG / TPT;
-  const size_t fl = (size_t)TPW * (g.Tw + g.Tw / 16) * sizeof(float2);
-  const size_t rb = ola_rowbuf_bytes(g, d);
-  return d.alias ? (fl > rb ? fl : rb) : fl + rb;
+  const size_t wl = d.win ? (size_t)d.mc * d.nslot * g.sw1 * sizeof(float) : 0;
+  return (size_t)ola_window_offset_floats(g, d) * sizeof(float) + wl;
 }
 
 int ola_single_round(const Geo& g, const OlaDst& d) {
