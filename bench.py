@@ -68,13 +68,17 @@ def kernel_models(plan, B):
     sw0 = plan.sw[0]
     sf0, sf1 = plan.sf
     ext0 = sw0 + sf0 - 1
+    nv = (plan.csf[0] - 1) * plan.nhop[0] + ext0
     sy0, sy1 = plan.sy
     rf, cf = fft_flops(Tw), fft_flops(Th)
     half = lambda r: (r + 1) // 2
     m = {
         'rows_r2c_multi': dict(bytes=B * (H * W * 4 + P * NB * sw0 * 8 + P * sf0 * sf1 * 4 + P * NB * sf0 * 8),
                                flops=B * P * (half(sw0) + half(sf0)) * rf),
-        'rows_r2c': dict(bytes=B * (sy0 * sy1 * 4 + P * NB * ext0 * 8), flops=B * P * half(ext0) * rf),
+        # residual rows shared by the patches of a patch column: one transform
+        # per (patch column, virtual row pair) over (csf0-1)*hop0 + ext0 rows
+        'rows_r2c': dict(bytes=B * (sy0 * sy1 * 4 + plan.csf[1] * NB * nv * 8),
+                         flops=B * plan.csf[1] * half(nv) * rf),
         'cols_fwd': dict(bytes=B * P * NB * (sw0 + sf0 + ext0) * 8, flops=B * P * NB * (3 * cf + 6 * Th)),
         # one inverse row transform per output row and pair of patch columns
         # (the covering patch rows are summed before the transform)

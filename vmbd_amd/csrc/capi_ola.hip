@@ -26,7 +26,7 @@ struct vmbd_plan {
   int* row_class = nullptr;  // [csf0][nslot] window class of each slot (-1 unused)
   int nslot = 0;
   int n_win = 0;
-  float2* wsG = nullptr;  // [P][NB][row_pitch(ext0)]
+  float2* wsG = nullptr;  // [frames][csf1][NB][row_pitch((csf0-1)*hop0 + ext0)] shared residual rows
   float2* wsX = nullptr;  // [P][NB][row_pitch(sw0)]
   float2* wsF = nullptr;  // [P][NB][row_pitch(sf0)]
   float2* Y1 = nullptr;   // [P][ext0][NBP]
@@ -148,9 +148,41 @@ RowSrc src_residual(const vmbd_plan* p, const float* gres, int exact) {
 }
 
 ColIn raw_in(const float2* ws, const RowSrc& s) {
-  return ColIn{ws, s.rlo, s.rhi - s.rlo, 0, row_pitch(s.rhi - s.rlo)};
+  return ColIn{ws, s.rlo, s.rhi - s.rlo, 0, row_pitch(s.rhi - s.rlo), 0};
 }
-ColIn spec_in(const void* spec, int Th) { return ColIn{(const float2*)spec, 0, Th, 1, Th}; }
+ColIn spec_in(const void* spec, int Th) { return ColIn{(const float2*)spec, 0, Th, 1, Th, 0}; }
+
+// Residual rows shared by the patches of a patch column.  Patch (m, n) reads
+// tile row b from residual row m*hop0 + b - oc0 over columns that depend on n
+// only, so the row transform of virtual row v = m*hop0 + b is the same for
+// every covering m: the row pass runs once per (frame, patch column) over
+// v in [rlo, rlo + (csf0-1)*hop0 + nr), on a geometry with one patch row
+// whose patches are the patch columns.
+struct SharedRows {
+  Geo g;      // row-pass geometry: csf0 = 1, Pf = csf1
+  RowSrc s;   // rows [rlo, rlo + nv)
+  int nr;     // rows of one patch
+  int nv;     // virtual rows of a patch column
+};
+
+int residual_virtual_rows(const Geo& g, int nr) { return (g.csf0 - 1) * g.hop0 + nr; }
+
+SharedRows residual_rows(const vmbd_plan* p, const Geo& g, const float* gres, int exact) {
+  SharedRows r;
+  r.s = src_residual(p, gres, exact);
+  r.nr = r.s.rhi - r.s.rlo;
+  r.nv = residual_virtual_rows(g, r.nr);
+  r.s.rhi = r.s.rlo + r.nv;
+  r.g = g;
+  r.g.csf0 = 1;
+  r.g.Pf = g.csf1;
+  r.g.P = (g.P / g.Pf) * g.csf1;
+  return r;
+}
+
+ColIn shared_in(const float2* ws, const SharedRows& r) {
+  return ColIn{ws, r.s.rlo, r.nr, 0, row_pitch(r.nv), 1};
+}
 
 float inv_scale(const Geo& g) { return 1.0f / (float)((double)g.Th * (double)g.Tw); }
 
@@ -350,7 +382,7 @@ vmbd_plan* vmbd_plan_create(const vmbd_geom* gm, const float* win_table, int32_t
   size_t tot = 0;
   const size_t P = (size_t)g.Pf * p->max_frames, NB = g.NB, NBP = g.NBP;
   bool ok = dev_alloc(&p->tw, TW_LEN, tot) && dev_alloc(&p->win, (size_t)n_win * g.sw0 * g.sw1, tot) &&
-            dev_alloc(&p->win_idx, g.Pf, tot) && dev_alloc(&p->wsG, P * NB * row_pitch(g.ext0), tot) &&
+            dev_alloc(&p->win_idx, g.Pf, tot) && dev_alloc(&p->wsG, (size_t)p->max_frames * g.csf1 * NB * row_pitch(residual_virtual_rows(g, g.ext0)), tot) &&
             dev_alloc(&p->wsX, P * NB * row_pitch(g.sw0), tot) && dev_alloc(&p->wsF, P * NB * row_pitch(g.sf0), tot) &&
             dev_alloc(&p->Y1, P * NBP * g.ext0, tot) && dev_alloc(&p->Y2, P * NBP * g.sf0, tot) &&
             dev_alloc(&p->win_slot, g.Pf, tot) && dev_alloc(&p->row_class, rclass.size(), tot);
@@ -424,13 +456,14 @@ int vmbd_ola_adjoint_x(vmbd_plan* p, int32_t nframes, const float* gres, const f
   hipStream_t st = (hipStream_t)stream;
   const Geo g = call_geo(p, nframes);
   p->rows_frames = 0;
-  RowSrc srcs[2] = {src_residual(p, gres, exact), src_psf(p, f)};
-  float2* outs[2] = {p->wsG, p->wsF};
-  VMBD_LAUNCH(launch_rows_r2c_multi(g, srcs, outs, 2, p->tw, st));
+  const SharedRows sr = residual_rows(p, g, gres, exact);
+  const RowSrc sf = src_psf(p, f);
+  VMBD_LAUNCH(launch_rows_r2c(sr.g, sr.s, p->wsG, p->tw, st));
+  VMBD_LAUNCH(launch_rows_r2c(g, sf, p->wsF, p->tw, st));
   ColOut o1{p->Y1, g.sw0}, o2{nullptr, 0};
-  ColIn none{nullptr, 0, 0, 0};
-  VMBD_LAUNCH(launch_cols(g, COL_ADJ1, raw_in(p->wsG, srcs[0]), raw_in(p->wsF, srcs[1]), none, o1, o2,
-                          inv_scale(g), nullptr, p->tw, st));
+  ColIn none{};
+  VMBD_LAUNCH(launch_cols(g, COL_ADJ1, shared_in(p->wsG, sr), raw_in(p->wsF, sf), none, o1, o2, inv_scale(g),
+                          nullptr, p->tw, st));
   VMBD_LAUNCH(launch_rows_c2r_ola(g, dst_adjoint_x(p, xbar), p->tw, st));
   return VMBD_OK;
 }
@@ -443,13 +476,14 @@ int vmbd_ola_adjoint_f(vmbd_plan* p, int32_t nframes, const float* gres, const f
   hipStream_t st = (hipStream_t)stream;
   const Geo g = call_geo(p, nframes);
   p->rows_frames = 0;
-  RowSrc srcs[2] = {src_residual(p, gres, exact), src_image(p, x)};
-  float2* outs[2] = {p->wsG, p->wsX};
-  VMBD_LAUNCH(launch_rows_r2c_multi(g, srcs, outs, 2, p->tw, st));
+  const SharedRows sr = residual_rows(p, g, gres, exact);
+  const RowSrc sx = src_image(p, x);
+  VMBD_LAUNCH(launch_rows_r2c(sr.g, sr.s, p->wsG, p->tw, st));
+  VMBD_LAUNCH(launch_rows_r2c(g, sx, p->wsX, p->tw, st));
   ColOut o1{p->Y2, g.sf0}, o2{nullptr, 0};
-  ColIn none{nullptr, 0, 0, 0};
-  VMBD_LAUNCH(launch_cols(g, COL_ADJ1, raw_in(p->wsG, srcs[0]), raw_in(p->wsX, srcs[1]), none, o1, o2,
-                          inv_scale(g), nullptr, p->tw, st));
+  ColIn none{};
+  VMBD_LAUNCH(launch_cols(g, COL_ADJ1, shared_in(p->wsG, sr), raw_in(p->wsX, sx), none, o1, o2, inv_scale(g),
+                          nullptr, p->tw, st));
   VMBD_LAUNCH(launch_rows_c2r_stack(g, p->Y2, g.sf0, fbar, g.sf0, g.sf1, p->tw, st));
   return VMBD_OK;
 }
@@ -466,18 +500,18 @@ int vmbd_ola_adjoint_xf(vmbd_plan* p, int32_t nframes, const float* gres, const 
   if (!reuse && (!x || !f)) return vmbd_fail(VMBD_E_ARG, "vmbd_ola_adjoint_xf: null x/f");
   hipStream_t st = (hipStream_t)stream;
   const Geo g = call_geo(p, nframes);
-  RowSrc sg = src_residual(p, gres, exact), sx = src_image(p, x), sf = src_psf(p, f);
-  if (reuse) {
-    VMBD_LAUNCH(launch_rows_r2c(g, sg, p->wsG, p->tw, st));
-  } else {
+  const SharedRows sr = residual_rows(p, g, gres, exact);
+  RowSrc sx = src_image(p, x), sf = src_psf(p, f);
+  VMBD_LAUNCH(launch_rows_r2c(sr.g, sr.s, p->wsG, p->tw, st));
+  if (!reuse) {
     p->rows_frames = 0;
-    RowSrc srcs[3] = {sg, sx, sf};
-    float2* outs[3] = {p->wsG, p->wsX, p->wsF};
-    VMBD_LAUNCH(launch_rows_r2c_multi(g, srcs, outs, 3, p->tw, st));
+    RowSrc srcs[2] = {sx, sf};
+    float2* outs[2] = {p->wsX, p->wsF};
+    VMBD_LAUNCH(launch_rows_r2c_multi(g, srcs, outs, 2, p->tw, st));
     p->rows_frames = nframes;  // wsX/wsF now hold the rows of this x and f
   }
   ColOut o1{p->Y1, g.sw0}, o2{p->Y2, g.sf0};
-  VMBD_LAUNCH(launch_cols(g, COL_ADJ2, raw_in(p->wsG, sg), raw_in(p->wsF, sf), raw_in(p->wsX, sx), o1, o2,
+  VMBD_LAUNCH(launch_cols(g, COL_ADJ2, shared_in(p->wsG, sr), raw_in(p->wsF, sf), raw_in(p->wsX, sx), o1, o2,
                           inv_scale(g), nullptr, p->tw, st));
   VMBD_LAUNCH(launch_rows_c2r_ola(g, dst_adjoint_x(p, xbar), p->tw, st));
   VMBD_LAUNCH(launch_rows_c2r_stack(g, p->Y2, g.sf0, fbar, g.sf0, g.sf1, p->tw, st));
@@ -564,18 +598,18 @@ int vmbd_ola_adjoint_cached(vmbd_plan* p, int32_t nframes, int32_t kind, const v
     return vmbd_fail(VMBD_E_ARG, "vmbd_ola_adjoint_cached: bad kind");
   hipStream_t st = (hipStream_t)stream;
   const Geo g = call_geo(p, nframes);
-  RowSrc sg = src_residual(p, gres, exact);
-  VMBD_LAUNCH(launch_rows_r2c(g, sg, p->wsG, p->tw, st));
-  ColIn none{nullptr, 0, 0, 0};
+  const SharedRows sr = residual_rows(p, g, gres, exact);
+  VMBD_LAUNCH(launch_rows_r2c(sr.g, sr.s, p->wsG, p->tw, st));
+  ColIn none{};
   ColOut o2{nullptr, 0};
   if (kind == VMBD_CACHE_PSF) {
     ColOut o1{p->Y1, g.sw0};
-    VMBD_LAUNCH(launch_cols(g, COL_ADJ1, raw_in(p->wsG, sg), spec_in(spec, g.Th), none, o1, o2, inv_scale(g),
+    VMBD_LAUNCH(launch_cols(g, COL_ADJ1, shared_in(p->wsG, sr), spec_in(spec, g.Th), none, o1, o2, inv_scale(g),
                             nullptr, p->tw, st));
     VMBD_LAUNCH(launch_rows_c2r_ola(g, dst_adjoint_x(p, out), p->tw, st));
   } else {
     ColOut o1{p->Y2, g.sf0};
-    VMBD_LAUNCH(launch_cols(g, COL_ADJ1, raw_in(p->wsG, sg), spec_in(spec, g.Th), none, o1, o2, inv_scale(g),
+    VMBD_LAUNCH(launch_cols(g, COL_ADJ1, shared_in(p->wsG, sr), spec_in(spec, g.Th), none, o1, o2, inv_scale(g),
                             nullptr, p->tw, st));
     VMBD_LAUNCH(launch_rows_c2r_stack(g, p->Y2, g.sf0, out, g.sf0, g.sf1, p->tw, st));
   }

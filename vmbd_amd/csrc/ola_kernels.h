@@ -58,10 +58,14 @@ __host__ __device__ inline int row_pitch(int nr) { return (nr + 15) & ~15; }
 
 // Column-pass input: either raw row-transform output (spec == 0, pitch =
 // row_pitch(nr)), or a cached spectrum laid out [P][NB][Th] (spec == 1,
-// pitch = Th).
+// pitch = Th).  shared == 1: the row transforms are shared by the patches of
+// one patch column, laid out [frame][csf1][NB][pitch] over virtual rows
+// v = m*hop0 + b (b in [rlo, rlo+nr)); patch (m, n) reads group
+// (frame, n) from row m*hop0.
 struct ColIn {
   const float2* p;
   int rlo, nr, spec, pitch;
+  int shared;
 };
 
 // Column-pass output rows [0, nrows) in layout [P][nrows][NBP].

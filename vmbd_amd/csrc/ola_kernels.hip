@@ -215,14 +215,16 @@ hipError_t launch_rows_r2c(const Geo& g, const RowSrc& s, float2* out, const flo
 // cols
 // ----------------------------------------------------------------------------
 template <int TH>
-__device__ __forceinline__ void col_load(float2 (&v)[16], const ColIn& in, int p, int c, int NB, int t, bool active) {
+__device__ __forceinline__ void col_load(float2 (&v)[16], const ColIn& in, int p, int c, int NB, int t, bool active,
+                                         int grp, int row0) {
   constexpr int TPT = TH / 16;
   if (in.spec) {
     const float2* col = in.p + ((size_t)p * NB + c) * TH;
 #pragma unroll
     for (int r = 0; r < 16; ++r) v[r] = active ? col[t + TPT * r] : make_float2(0.f, 0.f);
   } else {
-    const float2* col = in.p + ((size_t)p * NB + c) * in.pitch;
+    const float2* col = in.shared ? in.p + ((size_t)grp * NB + c) * in.pitch + row0
+                                  : in.p + ((size_t)p * NB + c) * in.pitch;
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
       const int rr = t + TPT * r - in.rlo;
@@ -272,13 +274,19 @@ __global__ __launch_bounds__(WG) void k_cols(Geo g, ColIn a, ColIn b, ColIn c3, 
   const bool active = c < NB;
   // a wavefront whose columns all lie past the last bin skips its transforms
   const bool wact = strip * CS + (int)(threadIdx.x >> 6) * (64 / TPT) < NB;
+  // group and row offset of this patch in shared row transforms
+  const int pfr = p / g.Pf;
+  const int pk = p - pfr * g.Pf;
+  const int pm = pk / g.csf1;
+  const int grp = pfr * g.csf1 + (pk - pm * g.csf1);
+  const int row0 = pm * g.hop0;
   float2* sl = sm + cc * SL;
 
   Twiddles<TH> T;
   twiddles_init<TH, true>(T, t, tw);
 
   float2 A[16];
-  col_load<TH>(A, a, p, c, NB, t, active);
+  col_load<TH>(A, a, p, c, NB, t, active, grp, row0);
   if (!a.spec && wact) fft<TH, -1>(A, sl, t, T, tw);
   if constexpr (OP == COL_SPEC) {
     if (active) {
@@ -288,7 +296,7 @@ __global__ __launch_bounds__(WG) void k_cols(Geo g, ColIn a, ColIn b, ColIn c3, 
     }
   } else {
     float2 B[16];
-    col_load<TH>(B, b, p, c, NB, t, active);
+    col_load<TH>(B, b, p, c, NB, t, active, grp, row0);
     if (!b.spec && wact) fft<TH, -1>(B, sl, t, T, tw);
 #pragma unroll
     for (int m = 0; m < 16; ++m)
@@ -296,7 +304,7 @@ __global__ __launch_bounds__(WG) void k_cols(Geo g, ColIn a, ColIn b, ColIn c3, 
     if (wact) fft<TH, +1>(B, sl, t, T, tw);
     col_store_rows<TH, CS>(B, sm, o1, p, strip, NB, g.NBP, cc, t);
     if constexpr (OP == COL_ADJ2) {
-      col_load<TH>(B, c3, p, c, NB, t, active);
+      col_load<TH>(B, c3, p, c, NB, t, active, grp, row0);
       if (!c3.spec && wact) fft<TH, -1>(B, sl, t, T, tw);
 #pragma unroll
       for (int m = 0; m < 16; ++m) B[m] = cscale(cmulc(A[m], B[m]), scale);
