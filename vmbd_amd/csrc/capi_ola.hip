@@ -369,8 +369,7 @@ vmbd_plan* vmbd_plan_create(const vmbd_geom* gm, const float* win_table, int32_t
   p->mc_fwd = max_cover(g.csf0, g.hop0, g.ext0, g.oc0, g.oc0 + g.sy0);
   p->mc_adj = max_cover(g.csf0, g.hop0, g.sw0, 0, g.H);
   {
-    const OlaDst df = dst_forward(p, nullptr);
-    const size_t s_fwd = vsum_shared_bytes(g, df, vsum_rows(g, df));
+    const size_t s_fwd = ola_shared_bytes(g, dst_forward(p, nullptr));
     const size_t s_adj = ola_shared_bytes(g, dst_adjoint_x(p, nullptr));
     if (s_fwd > 160 * 1024 || s_adj > 160 * 1024) {
       vmbd_fail(VMBD_E_UNSUP, "vmbd_plan_create: OLA gather needs %zu B of LDS > 160 KiB (csf1=%d ext1=%d)",

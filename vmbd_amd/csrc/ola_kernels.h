@@ -113,7 +113,5 @@ hipError_t launch_rows_c2r_stack(const Geo& g, const float2* src, int srow, floa
 size_t ola_rowbuf_bytes(const Geo& g, const OlaDst& d);
 size_t ola_shared_bytes(const Geo& g, const OlaDst& d);
 int ola_single_round(const Geo& g, const OlaDst& d);
-int vsum_rows(const Geo& g, const OlaDst& d);
-size_t vsum_shared_bytes(const Geo& g, const OlaDst& d, int R);
 
 }  // namespace vmbd

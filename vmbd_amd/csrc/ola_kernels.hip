@@ -390,128 +390,6 @@ __device__ __forceinline__ void c2r_load(float2 (&v)[16], const float2* ra, cons
 // ----------------------------------------------------------------------------
 __host__ __device__ int ola_window_offset_floats(const Geo& g, const OlaDst& d);
 
-// ----------------------------------------------------------------------------
-// rows_c2r_vsum: forward OLA (no window) with the vertical overlap summed in
-// the row-frequency domain.  For output row u and patch column n, every
-// covering patch row m places its row r_m = u - m*hop0 at the same columns
-// [n*hop1, n*hop1 + ext1), and the inverse row transform is linear:
-//   sum_m IFFT(Y1[k(m,n)][r_m]) = IFFT(sum_m Y1[k(m,n)][r_m]),
-// so one transform serves all covering patch rows (pairs of patch columns
-// share it as real and imaginary part).  A workgroup takes R consecutive
-// output rows so that its TPW transform slots are filled; the horizontal
-// overlap-add is the fixed-order LDS gather of the per-patch kernel.
-// ----------------------------------------------------------------------------
-template <int TW>
-__global__ __launch_bounds__(WG) void k_rows_c2r_vsum(Geo g, OlaDst d, int R, const float2* __restrict__ tw) {
-  constexpr int TPT = TW / 16;
-  constexpr int TPW = WG / TPT;
-  constexpr int SL = FftLds<TW>::floats2;
-  extern __shared__ float2 dyn[];
-  float2* fl = dyn;
-  float* rowbuf = d.alias ? (float*)dyn : (float*)(dyn + TPW * SL);
-
-  const int fr = blockIdx.y;
-  const int i0 = blockIdx.x * R;
-  const int kf = fr * g.Pf;
-  const int NP2 = (g.csf1 + 1) >> 1;
-  const int NT = R * NP2;
-  const int tr = threadIdx.x / TPT;
-  const int t = threadIdx.x - tr * TPT;
-  float2* sl = fl + tr * SL;
-
-  Twiddles<TW> T;
-  twiddles_init<TW>(T, t, tw);
-
-  for (int base = 0; base < NT; base += TPW) {
-    const int tix = base + tr;
-    const bool act = tix < NT;
-    const int rho = act ? tix / NP2 : 0;
-    const int jp = act ? tix - rho * NP2 : 0;
-    const int i = i0 + rho;
-    const bool rowok = act && i < d.O0;
-    const int u = i + d.oc0;
-    int mlo = 0, mhi = -1;
-    if (rowok && u >= 0) {
-      mhi = min(g.csf0 - 1, u / g.hop0);
-      const int lo = u - d.ext0 + 1;
-      mlo = lo <= 0 ? 0 : (lo + g.hop0 - 1) / g.hop0;
-    }
-    const int na = 2 * jp, nb = 2 * jp + 1;
-    const bool hasb = nb < g.csf1;
-    float2 v[16];
-#pragma unroll
-    for (int mm = 0; mm < 16; ++mm) v[mm] = make_float2(0.f, 0.f);
-    // a wavefront with no item in this round skips its transforms
-    const bool wact = base + (int)(threadIdx.x >> 6) * (64 / TPT) < NT;
-    if (wact) {
-      for (int mi = 0; mi < d.mc; ++mi) {
-        const int m = mlo + mi;
-        const int r = u - m * g.hop0;
-        const bool ok = m <= mhi && r >= 0 && r < d.srow;
-        const float2* ra = ok ? d.src + ((size_t)(kf + m * g.csf1 + na) * d.srow + r) * g.NBP : d.src;
-        const float2* rb = (ok && hasb) ? ra + (size_t)d.srow * g.NBP : ra;
-        float2 w[16];
-        c2r_load<TW>(w, ra, rb, ok && hasb, t);
-        const float sc = ok ? 1.f : 0.f;
-#pragma unroll
-        for (int mm = 0; mm < 16; ++mm) v[mm] = make_float2(v[mm].x + sc * w[mm].x, v[mm].y + sc * w[mm].y);
-      }
-      fft<TW, +1>(v, sl, t, T, tw);
-    }
-    if (d.alias) __syncthreads();  // slices are about to be overwritten by rowbuf
-    if (rowok) {
-      float* ba = rowbuf + (rho * g.csf1 + na) * d.ext1;
-      float* bb = rowbuf + (rho * g.csf1 + nb) * d.ext1;
-#pragma unroll
-      for (int mm = 0; mm < 16; ++mm) {
-        const int col = t + TPT * mm;
-        if (col < d.ext1) {
-          ba[col] = v[mm].x;
-          if (hasb) bb[col] = v[mm].y;
-        }
-      }
-    }
-  }
-  __syncthreads();
-  // fixed-order horizontal gather over the covering patch columns
-  const float inv_hop1 = 1.0f / (float)g.hop1;
-  for (int rho = 0; rho < R; ++rho) {
-    const int i = i0 + rho;
-    if (i >= d.O0) break;
-    const float* rb = rowbuf + rho * g.csf1 * d.ext1;
-    for (int j = threadIdx.x; j < d.O1; j += WG) {
-      const int vcol = j + d.oc1;
-      float acc = 0.f;
-      if (vcol >= 0) {
-        int qh = (int)((float)vcol * inv_hop1);
-        qh += ((qh + 1) * g.hop1 <= vcol) - (qh * g.hop1 > vcol);
-        const int nhi = min(g.csf1 - 1, qh);
-        for (int nn = nhi, lc = vcol - nhi * g.hop1; nn >= 0 && lc < d.ext1; --nn, lc += g.hop1)
-          acc += rb[nn * d.ext1 + lc];
-      }
-      d.out[((size_t)fr * d.O0 + i) * d.O1 + j] = acc;
-    }
-  }
-}
-
-// output rows per workgroup of rows_c2r_vsum: enough to fill the transform
-// slots, bounded so the gather buffer stays within 64 KiB
-int vsum_rows(const Geo& g, const OlaDst& d) {
-  const int TPW = WG / (g.Tw / 16);
-  const int NP2 = (g.csf1 + 1) / 2;
-  int R = TPW / NP2;
-  if (R < 1) R = 1;
-  while (R > 1 && (size_t)R * g.csf1 * d.ext1 * sizeof(float) > 64 * 1024) --R;
-  return R;
-}
-
-size_t vsum_shared_bytes(const Geo& g, const OlaDst& d, int R) {
-  const int TPW = WG / (g.Tw / 16);
-  const size_t fl = (size_t)TPW * (g.Tw + g.Tw / 16) * sizeof(float2);
-  const size_t rb = (size_t)R * g.csf1 * d.ext1 * sizeof(float);
-  const bool single = R * ((g.csf1 + 1) / 2) <= TPW;
-  return single ? (fl > rb ? fl : rb) : fl + rb;
-}
 
 template <int TW>
 __global__ __launch_bounds__(WG) void k_rows_c2r_ola(Geo g, OlaDst d, const float2* __restrict__ tw) {
@@ -660,19 +538,8 @@ int ola_single_round(const Geo& g, const OlaDst& d) {
 template <int TW>
 static hipError_t c2r_ola_t(const Geo& g, const OlaDst& d, const float2* tw, hipStream_t st) {
   const int frames = g.P / g.Pf;
-  if (!d.win) {
-    // forward: vertical overlap summed before the transform
-    const int R = vsum_rows(g, d);
-    OlaDst dv = d;
-    dv.alias = R * ((g.csf1 + 1) / 2) <= WG / (TW / 16);
-    const size_t shm = vsum_shared_bytes(g, dv, R);
-    ProfScope ps("rows_c2r_ola", st);
-    hipLaunchKernelGGL(k_rows_c2r_vsum<TW>, dim3((unsigned)((d.O0 + R - 1) / R), (unsigned)frames), dim3(WG), shm,
-                       st, g, dv, R, tw);
-    return hipGetLastError();
-  }
   const size_t shm = ola_shared_bytes(g, d);
-  ProfScope ps("rows_c2r_ola_win", st);
+  ProfScope ps(d.win ? "rows_c2r_ola_win" : "rows_c2r_ola", st);
   hipLaunchKernelGGL(k_rows_c2r_ola<TW>, dim3((unsigned)d.O0, (unsigned)frames), dim3(WG), shm, st, g, d, tw);
   return hipGetLastError();
 }
