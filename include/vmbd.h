@@ -141,6 +141,18 @@ int vmbd_ola_forward_cached(vmbd_plan* plan, int32_t nframes, int32_t kind, cons
 int vmbd_ola_adjoint_cached(vmbd_plan* plan, int32_t nframes, int32_t kind, const void* spec, const float* g,
                             float* out, int32_t exact, vmbd_stream_t stream);
 
+/* Solver forms of the cached operator (no counterpart in the reference,
+ * whose objectives form the residual and the Tikhonov term in separate
+ * PyCUDA passes, optGPU.py:33-46, olaGPU.py:756-811); the extra arithmetic
+ * is done in the final store of the operator:
+ *   forward_cached_ex: y = (B u - sub) * mul     (sub, mul may be NULL)
+ *   adjoint_cached_ex: out = B' g + beta * z     (z may be NULL)
+ * with sub, mul shaped like y and z like out. */
+int vmbd_ola_forward_cached_ex(vmbd_plan* plan, int32_t nframes, int32_t kind, const void* spec, const float* u,
+                               const float* sub, const float* mul, float* y, vmbd_stream_t stream);
+int vmbd_ola_adjoint_cached_ex(vmbd_plan* plan, int32_t nframes, int32_t kind, const void* spec, const float* g,
+                               const float* z, float beta, float* out, int32_t exact, vmbd_stream_t stream);
+
 /* Opt-in per-kernel timing (no counterpart in the reference, whose only
  * timing is wall-clock prints, stopwatch.py:5-22).  When enabled, every kernel
  * launch is bracketed by HIP events on its stream; vmbd_prof_report()

@@ -258,3 +258,30 @@ def test_named_entry_points(cuda):
         _lib.call('vmbd_pbb_step', _lib.ptr(o2), _lib.ptr(a), _lib.ptr(b), _lib.ptr(c), _lib.ptr(d), n, it, 0,
                   _lib.ptr(part), _lib.ptr(s2), _lib.stream_ptr())
         assert torch.equal(o1, o2) and torch.equal(s1, s2), it
+
+
+@pytest.mark.parametrize('exact', [False, True])
+def test_fused_solver_forms(cuda, exact):
+    """forward_cached(sub, mul) = (B u - sub) * mul and adjoint_cached(z, beta)
+    = B' g + beta z, for both cache kinds."""
+    import torch
+    from vmbd_amd.imagetools import win2winaux
+    from vmbd_amd.ola import OlaPlan
+    print(f'case fused solver forms exact={exact}', flush=True)
+    sx, csf, sf = (96, 96), (3, 3), (9, 9)
+    plan = OlaPlan(sx, sf, win2winaux(sx, csf, 0.5), 'valid')
+    x, f, g = case_inputs(90, sx, csf, sf)
+    t = lambda a: torch.as_tensor(np.asarray(a, dtype=np.float32), device=cuda)
+    xt, ft, gt = t(x), t(f), t(g)
+    rng = np.random.default_rng(91)
+    sub = t(rng.standard_normal(plan.sy))
+    mul = t(rng.random(plan.sy) < 0.8)
+    for kind, spec, u, zshape in (('F', plan.spectra('F', ft), xt, sx), ('X', plan.spectra('X', xt), ft, f.shape)):
+        y = plan.forward_cached(kind, spec, u)
+        assert torch.equal(plan.forward_cached(kind, spec, u, sub=sub, mul=mul), (y - sub) * mul), kind
+        assert torch.equal(plan.forward_cached(kind, spec, u, sub=sub), y - sub), kind
+        z = t(rng.standard_normal(zshape))
+        a = plan.adjoint_cached(kind, spec, gt, exact=exact)
+        b = plan.adjoint_cached(kind, spec, gt, exact=exact, z=z, beta=0.37)
+        ref = (a.double() + 0.37 * z.double()).cpu().numpy()
+        assert rel_l2(b.cpu().numpy(), ref) < 1e-6, kind

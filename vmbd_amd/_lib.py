@@ -52,6 +52,8 @@ SIGNATURES = {
     'vmbd_windows': (ctypes.c_int, [_P, _F, _P]),
     'vmbd_ola_forward_cached': (ctypes.c_int, [_P, _I, _I, _P, _F, _F, _P]),
     'vmbd_ola_adjoint_cached': (ctypes.c_int, [_P, _I, _I, _P, _F, _F, _I, _P]),
+    'vmbd_ola_forward_cached_ex': (ctypes.c_int, [_P, _I, _I, _P, _F, _F, _F, _F, _P]),
+    'vmbd_ola_adjoint_cached_ex': (ctypes.c_int, [_P, _I, _I, _P, _F, _F, ctypes.c_float, _F, _I, _P]),
     # plain (reference-decomposition) kernels, include/vmbd_plain.h
     'vmbd_pad': (ctypes.c_int, [_F, _I, _I, _F, _I, _I, _I, _I, _I, _P]),
     'vmbd_chop_pad': (ctypes.c_int, [_F, _I, _I, _F, _I, _I, _I, _I, _I, _I, _I, _I, _I, _I, _F, _F, _I, _P]),

@@ -566,8 +566,8 @@ int vmbd_spectra(vmbd_plan* p, int32_t nframes, int32_t kind, const float* z, vo
   return VMBD_OK;
 }
 
-int vmbd_ola_forward_cached(vmbd_plan* p, int32_t nframes, int32_t kind, const void* spec, const float* u, float* y,
-                            vmbd_stream_t stream) {
+int vmbd_ola_forward_cached_ex(vmbd_plan* p, int32_t nframes, int32_t kind, const void* spec, const float* u,
+                               const float* sub, const float* mul, float* y, vmbd_stream_t stream) {
   vmbd_clear_error();
   if (int r = check_plan(p, nframes)) return r;
   if (!spec || !u || !y) return vmbd_fail(VMBD_E_ARG, "vmbd_ola_forward_cached: null pointer");
@@ -581,15 +581,23 @@ int vmbd_ola_forward_cached(vmbd_plan* p, int32_t nframes, int32_t kind, const v
   float2* ws = kind == VMBD_CACHE_PSF ? p->wsX : p->wsF;
   VMBD_LAUNCH(launch_rows_r2c(g, s, ws, p->tw, st));
   ColOut o1{p->Y1, g.ext0}, o2{nullptr, 0};
-  ColIn none{nullptr, 0, 0, 0};
+  ColIn none{};
   VMBD_LAUNCH(launch_cols(g, COL_FWD, raw_in(ws, s), spec_in(spec, g.Th), none, o1, o2, inv_scale(g), nullptr, p->tw,
                           st));
-  VMBD_LAUNCH(launch_rows_c2r_ola(g, dst_forward(p, y), p->tw, st));
+  OlaDst d = dst_forward(p, y);
+  d.sub = sub;
+  d.mul = mul;
+  VMBD_LAUNCH(launch_rows_c2r_ola(g, d, p->tw, st));
   return VMBD_OK;
 }
 
-int vmbd_ola_adjoint_cached(vmbd_plan* p, int32_t nframes, int32_t kind, const void* spec, const float* gres,
-                            float* out, int32_t exact, vmbd_stream_t stream) {
+int vmbd_ola_forward_cached(vmbd_plan* p, int32_t nframes, int32_t kind, const void* spec, const float* u, float* y,
+                            vmbd_stream_t stream) {
+  return vmbd_ola_forward_cached_ex(p, nframes, kind, spec, u, nullptr, nullptr, y, stream);
+}
+
+int vmbd_ola_adjoint_cached_ex(vmbd_plan* p, int32_t nframes, int32_t kind, const void* spec, const float* gres,
+                               const float* z, float beta, float* out, int32_t exact, vmbd_stream_t stream) {
   vmbd_clear_error();
   if (int r = check_plan(p, nframes)) return r;
   if (!spec || !gres || !out) return vmbd_fail(VMBD_E_ARG, "vmbd_ola_adjoint_cached: null pointer");
@@ -605,14 +613,22 @@ int vmbd_ola_adjoint_cached(vmbd_plan* p, int32_t nframes, int32_t kind, const v
     ColOut o1{p->Y1, g.sw0};
     VMBD_LAUNCH(launch_cols(g, COL_ADJ1, shared_in(p->wsG, sr), spec_in(spec, g.Th), none, o1, o2, inv_scale(g),
                             nullptr, p->tw, st));
-    VMBD_LAUNCH(launch_rows_c2r_ola(g, dst_adjoint_x(p, out), p->tw, st));
+    OlaDst d = dst_adjoint_x(p, out);
+    d.axz = z;
+    d.beta = beta;
+    VMBD_LAUNCH(launch_rows_c2r_ola(g, d, p->tw, st));
   } else {
     ColOut o1{p->Y2, g.sf0};
     VMBD_LAUNCH(launch_cols(g, COL_ADJ1, shared_in(p->wsG, sr), spec_in(spec, g.Th), none, o1, o2, inv_scale(g),
                             nullptr, p->tw, st));
-    VMBD_LAUNCH(launch_rows_c2r_stack(g, p->Y2, g.sf0, out, g.sf0, g.sf1, p->tw, st));
+    VMBD_LAUNCH(launch_rows_c2r_stack(g, p->Y2, g.sf0, out, g.sf0, g.sf1, p->tw, st, z, beta));
   }
   return VMBD_OK;
+}
+
+int vmbd_ola_adjoint_cached(vmbd_plan* p, int32_t nframes, int32_t kind, const void* spec, const float* gres,
+                            float* out, int32_t exact, vmbd_stream_t stream) {
+  return vmbd_ola_adjoint_cached_ex(p, nframes, kind, spec, gres, nullptr, 0.f, out, exact, stream);
 }
 
 }  // extern "C"

@@ -96,6 +96,12 @@ struct OlaDst {
   int nslot;
   int mc;     // max number of patch rows covering one output row
   int alias;  // 1: all transforms of a row fit one round; gather buffer aliases the FFT slices
+  // fused solver arithmetic on the stored value v (each optional):
+  // v = ((v - sub[o]) * mul[o]) + beta * axz[o], o = output index
+  const float* sub;
+  const float* mul;
+  const float* axz;
+  float beta;
 };
 
 enum ColOp { COL_SPEC = 0, COL_FWD = 1, COL_ADJ1 = 2, COL_ADJ2 = 3 };
@@ -108,8 +114,9 @@ hipError_t launch_rows_r2c_multi(const Geo& g, const RowSrc* s, float2* const* o
 hipError_t launch_cols(const Geo& g, int op, const ColIn& a, const ColIn& b, const ColIn& c, const ColOut& o1,
                        const ColOut& o2, float scale, float2* spec_out, const float2* tw, hipStream_t st);
 hipError_t launch_rows_c2r_ola(const Geo& g, const OlaDst& d, const float2* tw, hipStream_t st);
+// out = inverse row transforms (+ beta * axz when axz != null)
 hipError_t launch_rows_c2r_stack(const Geo& g, const float2* src, int srow, float* out, int nrows, int ncols,
-                                 const float2* tw, hipStream_t st);
+                                 const float2* tw, hipStream_t st, const float* axz = nullptr, float beta = 0.f);
 size_t ola_rowbuf_bytes(const Geo& g, const OlaDst& d);
 size_t ola_shared_bytes(const Geo& g, const OlaDst& d);
 int ola_single_round(const Geo& g, const OlaDst& d);

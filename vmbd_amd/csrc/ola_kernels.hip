@@ -507,7 +507,11 @@ __global__ __launch_bounds__(WG) void k_rows_c2r_ola(Geo g, OlaDst d, const floa
         acc += sub;
       }
     }
-    d.out[((size_t)fr * d.O0 + i) * d.O1 + j] = acc;
+    const size_t o = ((size_t)fr * d.O0 + i) * d.O1 + j;
+    if (d.sub) acc -= d.sub[o];
+    if (d.mul) acc *= d.mul[o];
+    if (d.axz) acc += d.beta * d.axz[o];
+    d.out[o] = acc;
   }
 }
 
@@ -563,7 +567,8 @@ hipError_t launch_rows_c2r_ola(const Geo& g, const OlaDst& d, const float2* tw, 
 template <int TW>
 __global__ __launch_bounds__(WG) void k_rows_c2r_stack(Geo g, const float2* __restrict__ src, int srow,
                                                        float* __restrict__ out, int nrows, int ncols,
-                                                       const float2* __restrict__ tw) {
+                                                       const float2* __restrict__ tw, const float* __restrict__ axz,
+                                                       float beta) {
   constexpr int TPT = TW / 16;
   constexpr int TPW = WG / TPT;
   constexpr int SL = FftLds<TW>::floats2;
@@ -591,8 +596,12 @@ __global__ __launch_bounds__(WG) void k_rows_c2r_stack(Geo g, const float2* __re
     for (int mm = 0; mm < 16; ++mm) {
       const int col = t + TPT * mm;
       if (col < ncols) {
-        out[((size_t)p * nrows + r0) * ncols + col] = v[mm].x;
-        if (r1 < nrows) out[((size_t)p * nrows + r1) * ncols + col] = v[mm].y;
+        const size_t o0 = ((size_t)p * nrows + r0) * ncols + col;
+        out[o0] = axz ? v[mm].x + beta * axz[o0] : v[mm].x;
+        if (r1 < nrows) {
+          const size_t o1 = o0 + ncols;
+          out[o1] = axz ? v[mm].y + beta * axz[o1] : v[mm].y;
+        }
       }
     }
   }
@@ -600,25 +609,25 @@ __global__ __launch_bounds__(WG) void k_rows_c2r_stack(Geo g, const float2* __re
 
 template <int TW>
 static hipError_t c2r_stack_t(const Geo& g, const float2* src, int srow, float* out, int nrows, int ncols,
-                              const float2* tw, hipStream_t st) {
+                              const float2* tw, hipStream_t st, const float* axz, float beta) {
   constexpr int TPW = WG / (TW / 16);
   const int blocks = ((nrows + 1) / 2 + TPW - 1) / TPW;
   if (blocks <= 0 || g.P <= 0) return hipSuccess;
   ProfScope ps("rows_c2r_stack", st);
   hipLaunchKernelGGL(k_rows_c2r_stack<TW>, dim3((unsigned)g.P, (unsigned)blocks), dim3(WG), 0, st, g, src, srow,
-                     out, nrows, ncols, tw);
+                     out, nrows, ncols, tw, axz, beta);
   return hipGetLastError();
 }
 
 hipError_t launch_rows_c2r_stack(const Geo& g, const float2* src, int srow, float* out, int nrows, int ncols,
-                                 const float2* tw, hipStream_t st) {
+                                 const float2* tw, hipStream_t st, const float* axz, float beta) {
   switch (g.Tw) {
-    case 16: return c2r_stack_t<16>(g, src, srow, out, nrows, ncols, tw, st);
-    case 32: return c2r_stack_t<32>(g, src, srow, out, nrows, ncols, tw, st);
-    case 64: return c2r_stack_t<64>(g, src, srow, out, nrows, ncols, tw, st);
-    case 128: return c2r_stack_t<128>(g, src, srow, out, nrows, ncols, tw, st);
-    case 256: return c2r_stack_t<256>(g, src, srow, out, nrows, ncols, tw, st);
-    case 512: return c2r_stack_t<512>(g, src, srow, out, nrows, ncols, tw, st);
+    case 16: return c2r_stack_t<16>(g, src, srow, out, nrows, ncols, tw, st, axz, beta);
+    case 32: return c2r_stack_t<32>(g, src, srow, out, nrows, ncols, tw, st, axz, beta);
+    case 64: return c2r_stack_t<64>(g, src, srow, out, nrows, ncols, tw, st, axz, beta);
+    case 128: return c2r_stack_t<128>(g, src, srow, out, nrows, ncols, tw, st, axz, beta);
+    case 256: return c2r_stack_t<256>(g, src, srow, out, nrows, ncols, tw, st, axz, beta);
+    case 512: return c2r_stack_t<512>(g, src, srow, out, nrows, ncols, tw, st, axz, beta);
     default: return hipErrorInvalidValue;
   }
 }

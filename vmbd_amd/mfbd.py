@@ -149,13 +149,12 @@ class OnlineMFBD:
         _lib.call('vmbd_dot', _lib.ptr(a), _lib.ptr(b), a.numel(), _lib.ptr(self.dot_part), out, _lib.stream_ptr())
 
     def _grad_f(self, z, out, dots_at=None):
-        """out = X'(X z - y) + beta z (X: cached image spectra)."""
+        """out = X'(X z - y) + beta z (X: cached image spectra); the residual
+        and the Tikhonov term are formed in the operators' final stores."""
         p = self.plan
-        p.forward_cached('X', self.specX, z, out=self.r)
-        self._ew(gputools.EW_SUB, self.r, self.r, self.y)
-        p.adjoint_cached('X', self.specX, self.r, exact=self.exact, out=out)
-        if self.f_beta != 0.:
-            self._ew(gputools.EW_AXPY, out, out, z, self.f_beta)
+        p.forward_cached('X', self.specX, z, out=self.r, sub=self.y)
+        p.adjoint_cached('X', self.specX, self.r, exact=self.exact, out=out,
+                         z=z if self.f_beta != 0. else None, beta=self.f_beta)
         if dots_at is not None:
             self._dot(self.r, self.r, dots_at)
             self._dot(z, z, dots_at + 1)
@@ -163,9 +162,7 @@ class OnlineMFBD:
     def _grad_x(self, z, out, dots_at=None):
         """out = F'(mask (F z - y)) (F: cached PSF spectra)."""
         p = self.plan
-        p.forward_cached('F', self.specF, z, out=self.r)
-        self._ew(gputools.EW_SUB, self.r, self.r, self.y)
-        self._ew(gputools.EW_MUL, self.r, self.r, self.mask)
+        p.forward_cached('F', self.specF, z, out=self.r, sub=self.y, mul=self.mask)
         p.adjoint_cached('F', self.specF, self.r, exact=self.exact, out=out)
         if dots_at is not None:
             self._dot(self.r, self.r, dots_at)
@@ -214,8 +211,7 @@ class OnlineMFBD:
         p = self.plan
         p.spectra('F', self.f_dev, out=self.specF)
         if self.x_solver == 'rl':
-            p.forward_cached('F', self.specF, self.x, out=self.r)
-            self._ew(gputools.EW_MUL, self.r, self.r, self.mask)
+            p.forward_cached('F', self.specF, self.x, out=self.r, mul=self.mask)
             p.adjoint_cached('F', self.specF, self.ym, exact=self.exact, out=self.num)
             p.adjoint_cached('F', self.specF, self.r, exact=self.exact, out=self.den)
             _lib.call('vmbd_rl_update', _lib.ptr(self.x), _lib.ptr(self.num), _lib.ptr(self.den), self.sx[0],

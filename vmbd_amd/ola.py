@@ -233,7 +233,17 @@ class OlaPlan:
             raise ValueError('spectrum tensor does not match the plan')
         return nf
 
-    def forward_cached(self, kind, spec, u, out=None):
+    def _like(self, t, shape, name):
+        if t is None:
+            return None
+        t = _as_dev(t, self.device, name)
+        if tuple(t.shape) != tuple(shape):
+            raise ValueError(f'{name} must have shape {tuple(shape)}')
+        return t
+
+    def forward_cached(self, kind, spec, u, out=None, sub=None, mul=None):
+        """B u with the cached operand; with `sub`/`mul` (shaped like y) the
+        result is (B u - sub) * mul, formed in the operator's final store."""
         if kind == 'F':
             u, nf, bat = self._x(u)
             k = _lib.CACHE_PSF
@@ -242,23 +252,31 @@ class OlaPlan:
             k = _lib.CACHE_IMAGE
         if self._spec_frames(spec) != nf:
             raise ValueError('spectrum and operand disagree on the number of frames')
-        y = self._empty(self._lead(nf, bat) + self.sy, out)
+        shape = self._lead(nf, bat) + self.sy
+        sub, mul = self._like(sub, shape, 'sub'), self._like(mul, shape, 'mul')
+        y = self._empty(shape, out)
         self._rows_key = None
-        _lib.check(_lib.lib().vmbd_ola_forward_cached(self._h, nf, k, _lib.ptr(spec), _lib.ptr(u), _lib.ptr(y),
-                                                      _lib.stream_ptr()), 'forward_cached')
+        _lib.check(_lib.lib().vmbd_ola_forward_cached_ex(self._h, nf, k, _lib.ptr(spec), _lib.ptr(u), _lib.ptr(sub),
+                                                         _lib.ptr(mul), _lib.ptr(y), _lib.stream_ptr()),
+                   'forward_cached')
         return y
 
-    def adjoint_cached(self, kind, spec, g, exact=False, out=None):
+    def adjoint_cached(self, kind, spec, g, exact=False, out=None, z=None, beta=0.0):
+        """B' g with the cached operand; with `z` (shaped like the result) the
+        result is B' g + beta z, formed in the operator's final store."""
         g, nf, bat = self._g(g)
         if self._spec_frames(spec) != nf:
             raise ValueError('spectrum and residual disagree on the number of frames')
         if kind == 'F':
             k = _lib.CACHE_PSF
-            o = self._empty(self._lead(nf, bat) + self.sx, out)
+            shape = self._lead(nf, bat) + self.sx
         else:
             k = _lib.CACHE_IMAGE
-            o = self._empty(self._lead(nf, bat) + (self.P,) + self.sf, out)
+            shape = self._lead(nf, bat) + (self.P,) + self.sf
+        z = self._like(z, shape, 'z')
+        o = self._empty(shape, out)
         self._rows_key = None
-        _lib.check(_lib.lib().vmbd_ola_adjoint_cached(self._h, nf, k, _lib.ptr(spec), _lib.ptr(g), _lib.ptr(o),
-                                                      int(bool(exact)), _lib.stream_ptr()), 'adjoint_cached')
+        _lib.check(_lib.lib().vmbd_ola_adjoint_cached_ex(self._h, nf, k, _lib.ptr(spec), _lib.ptr(g), _lib.ptr(z),
+                                                         float(beta), _lib.ptr(o), int(bool(exact)),
+                                                         _lib.stream_ptr()), 'adjoint_cached')
         return o
