@@ -325,12 +325,13 @@ class OlaOracle:
         """Residual patches G_k placed in the FFT tile.
 
         exact=False: olaGPU.py:276-286 -- chop y (padded to sxp) with patch
-        size sw at tile offset oc (= sf-1 valid / floor(sf/2) same).
+        size sw at tile offset oc (= sf-1 valid / floor(sf/2) same / 0 full;
+        the reference stops at valid/same, full extends the same rule).
         exact=True : the full correlation support, tile b in [0, sw+sf-1)
         holds y[p_k + b - oc] (zero outside the image).
         """
         wa = self.winaux
-        if self.mode not in ('valid', 'same'):
+        if self.mode not in ('valid', 'same', 'full'):
             raise NotImplementedError('Not a valid mode!')
         oc = crop_offset(self.sf, self.mode)
         y = np.asarray(y, dtype=np.float64)

@@ -219,9 +219,34 @@ def test_olagpu_cnv_cnvtp(cuda, small_case):
         F.cnv(np.ones((90, 96), np.float32))
     with pytest.raises(IOError):
         F.deconv(np.ones((10, 10), np.float32))
+    # full mode: the reference stops at valid/same for cnvtp; here it extends
+    # the same rule (residual patches at tile offset 0)
     Ff = OlaGPU(f.astype(np.float32), sx, 'full', wa)
+    gf = np.random.default_rng(42).standard_normal(tuple(int(v) for v in Ff.sy))
+    for exact in (False, True):
+        assert rel_l2(_n(Ff.cnvtp(gf.astype(np.float32), exact=exact)),
+                      O.adjoint_x(_f32(gf), _f32(f), owa, sx, 'full', exact=exact)) < 1e-5
+    Fc = OlaGPU(f.astype(np.float32), sx, 'circ', wa)
     with pytest.raises(NotImplementedError):
-        Ff.cnvtp(np.ones(Ff.sy, np.float32))
+        Fc.cnvtp(np.ones(Fc.sy, np.float32))
+
+
+def test_olagpu_deconv_rgb(cuda, small_case):
+    """olaGPU.py:739-753 with its keyword arguments in the right places:
+    every channel deconvolved on its own."""
+    from vmbd_amd.imagetools import win2winaux
+    from vmbd_amd.olaGPU import OlaGPU
+    sx, csf, sf, x, f, g = small_case
+    print('case OlaGPU deconv_rgb', flush=True)
+    wa = win2winaux(sx, csf, 0.5)
+    F = OlaGPU(f.astype(np.float32), sx, 'valid', wa)
+    sy = tuple(int(v) for v in F.sy)
+    rgb = np.random.default_rng(43).random(sy + (3,)).astype(np.float32)
+    out = F.deconv_rgb(rgb, mode='xdirect', alpha=0.05, beta=0.01)
+    assert out.shape == tuple(int(v) for v in F.sx) + (3,)
+    for c in range(3):
+        one = _n(F.deconv(np.ascontiguousarray(rgb[..., c]), mode='xdirect', alpha=0.05, beta=0.01))
+        np.testing.assert_array_equal(out[..., c], one.astype(np.float32))
 
 
 @pytest.mark.parametrize('mode', ['xdirect', 'direct', 'gdirect', 'sparse'])

@@ -330,6 +330,22 @@ def test_reference_errors():
         O.OlaOracle(np.ones((4, 5, 5)), (64, 64), 'bogus', wa)   # olaGPU.py:136
     with pytest.raises(IOError):
         O.OlaOracle(np.ones((4, 5, 5)), (60, 64), 'valid', wa)   # olaGPU.py:124-125
+    # olaGPU.py:285-286 stops cnvtp at valid/same; full extends the same rule
     F = O.OlaOracle(np.ones((4, 5, 5)) / 25, (64, 64), 'full', wa)
-    with pytest.raises(NotImplementedError):
-        F.cnvtp(np.ones((68, 68)))                               # olaGPU.py:285-286
+    assert F.cnvtp(np.ones((68, 68))).shape == (64, 64)
+
+
+def test_full_mode_transposes_dot_product():
+    """Full-mode exact transposes satisfy <Fx, g> = <x, F'g> = <f, X'g>."""
+    sx, csf, sf = (40, 48), (2, 3), (5, 7)
+    wa = O.Win2WinAux(sx, csf, 0.5)
+    rng = np.random.default_rng(17)
+    x = rng.random(sx)
+    f = rng.random((6,) + sf)
+    y = O.forward(x, f, wa, 'full')
+    g = rng.standard_normal(y.shape)
+    lhs = np.sum(y * g)
+    xb = O.adjoint_x(g, f, wa, sx, 'full', exact=True)
+    fb = O.adjoint_f(g, x, wa, sf, 'full', exact=True)
+    assert abs(lhs - np.sum(x * xb)) < 1e-12 * abs(lhs)
+    assert abs(lhs - np.sum(f * fb)) < 1e-12 * abs(lhs)

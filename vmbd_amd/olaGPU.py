@@ -143,8 +143,10 @@ class OlaGPU:
         return y.cpu().numpy() if host else y
 
     def cnvtp(self, y, exact=None):
-        """olaGPU.py:261-311: X' y (PSF shaped) or F' y (image shaped)."""
-        if self.mode not in ('valid', 'same'):
+        """olaGPU.py:261-311: X' y (PSF shaped) or F' y (image shaped).  The
+        reference stops at valid/same; 'full' extends the same rule (residual
+        patches at tile offset 0)."""
+        if self.mode not in ('valid', 'same', 'full'):
             raise NotImplementedError('Not a valid mode!')
         ex = self.exact if exact is None else bool(exact)
         return self.plan.adjoint_cached(self.__id__, self.fft_gpu, gputools._dev(y), exact=ex)
