@@ -423,45 +423,19 @@ __global__ __launch_bounds__(WG) void k_rows_c2r_ola(Geo g, OlaDst d, const floa
   twiddles_init<TW>(T, t, tw);
 
   // x-adjoint: the window rows this output row needs, one per (covering patch
-  // row, window class of that row), staged in LDS past the gather buffer.
-  // Wavefront w stages pairs w, w+4, ...; lane l columns l, l+64, ...  When a
-  // thread's share fits WPP x WCH registers, the values are loaded here and
-  // written to LDS after the first round's transforms, so their latency
-  // overlaps the row loads and the barrier after the transforms covers them.
-  constexpr int WPP = 3, WCH = 4;
-  constexpr bool LATE_OK = TW == 256;  // other tiles have no registers to spare
+  // row, window class of that row), staged in LDS past the gather buffer
   float* wl = (float*)(dyn) + ola_window_offset_floats(g, d);
-  const int npair = d.win ? Mc * d.nslot : 0;
-  const int wave = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
-  const bool wlate = LATE_OK && d.win && (npair + 3) / 4 <= WPP && g.sw1 <= 64 * WCH;
-  float wv[WPP * WCH];
-  if constexpr (LATE_OK) {
-#pragma unroll
-    for (int a = 0; a < WPP; ++a) {
-#pragma unroll
-      for (int b = 0; b < WCH; ++b) {
-        const int pr = wave + 4 * a, c = lane + 64 * b;
-        float val = 0.f;
-        if (wlate && pr < npair && c < g.sw1) {
-          const int mi = pr / d.nslot;
-          const int m = mlo + mi;
-          const int r = u - m * g.hop0;
-          const int cls = d.row_class[m * d.nslot + (pr - mi * d.nslot)];
-          if (cls >= 0 && r >= 0 && r < g.sw0) val = d.win[((size_t)cls * g.sw0 + r) * g.sw1 + c];
-        }
-        wv[a * WCH + b] = val;
-      }
-    }
-  }
-  if (d.win && !wlate) {
-    for (int pr = wave; pr < npair; pr += WG / 64) {
+  if (d.win) {
+    const int npair = Mc * d.nslot;
+    for (int pr = (int)(threadIdx.x >> 6); pr < npair; pr += WG / 64) {
       const int mi = pr / d.nslot;
+      const int sl = pr - mi * d.nslot;
       const int m = mlo + mi;
       const int r = u - m * g.hop0;
-      const int cls = d.row_class[m * d.nslot + (pr - mi * d.nslot)];
+      const int cls = d.row_class[m * d.nslot + sl];
       const bool ok = cls >= 0 && r >= 0 && r < g.sw0;
       const float* src = ok ? d.win + ((size_t)cls * g.sw0 + r) * g.sw1 : d.win;
-      for (int c = lane; c < g.sw1; c += 64) wl[pr * g.sw1 + c] = ok ? src[c] : 0.f;
+      for (int c = threadIdx.x & 63; c < g.sw1; c += 64) wl[pr * g.sw1 + c] = ok ? src[c] : 0.f;
     }
     __syncthreads();
   }
@@ -493,19 +467,7 @@ __global__ __launch_bounds__(WG) void k_rows_c2r_ola(Geo g, OlaDst d, const floa
 #pragma unroll
       for (int mm = 0; mm < 16; ++mm) v[mm] = make_float2(0.f, 0.f);
     }
-    if (LATE_OK && wlate && base == 0) {
-#pragma unroll
-      for (int a = 0; a < WPP; ++a) {
-#pragma unroll
-        for (int b = 0; b < WCH; ++b) {
-          const int pr = wave + 4 * a, c = lane + 64 * b;
-          if (pr < npair && c < g.sw1) wl[pr * g.sw1 + c] = wv[a * WCH + b];
-        }
-      }
-    }
-    // slices are about to be overwritten by rowbuf (single round); staged
-    // windows become visible
-    if (d.alias || (wlate && base == 0)) __syncthreads();
+    if (d.alias) __syncthreads();  // slices are about to be overwritten by rowbuf
     if (act) {
       float* ba = rowbuf + (mi * g.csf1 + na) * d.ext1;
       float* bb = rowbuf + (mi * g.csf1 + nb) * d.ext1;
