@@ -46,7 +46,7 @@ __global__ __launch_bounds__(WG) void k_rows_r2c(Geo g, RowJobs jobs, const floa
   constexpr int TPW = WG / TPT;
   constexpr int NBN = TW / 2 + 1;
   constexpr int SL = FftLds<TW>::floats2;
-  __shared__ float2 sm[cmax<TPW * SL, 2 * TPW * NBN>::v];
+  __shared__ float2 sm[TPW * SL];
 
   // 1-D grid over work units (patch, row block), row blocks of all jobs
   // job-major.  Workgroup ids are dealt round-robin to the NXCD XCDs; XCD j
@@ -145,30 +145,19 @@ __global__ __launch_bounds__(WG) void k_rows_r2c(Geo g, RowJobs jobs, const floa
 #pragma unroll
     for (int mm = 0; mm <= 8; ++mm) za[mm] = zb[mm] = make_float2(0.f, 0.f);
   }
-  __syncthreads();  // every slice has been read: the buffer becomes staging
+  // bin-major store without staging: bin c of rows 2q (A) and 2q+1 (B) is
+  // one 16-byte write; the four row pairs of a wavefront cover 64 contiguous
+  // bytes of each bin and the workgroup's 32 rows (row blocks start on
+  // 32-row, 256-byte boundaries of the 16-row pitch) two whole lines, which
+  // fill in L2 before they are written back
+  if (active) {
+    const int nrp = row_pitch(nr);
+    float2* dst = out + (size_t)p * NBN * nrp + 2 * q;
 #pragma unroll
-  for (int mm = 0; mm <= 8; ++mm) {
-    const int c = t + TPT * mm;
-    if (c < NBN) {
-      sm[(2 * tr) * NBN + c] = za[mm];
-      sm[(2 * tr + 1) * NBN + c] = zb[mm];
-    }
-  }
-  __syncthreads();
-  // bin-major store: consecutive lanes -> consecutive rows of one bin; the
-  // block's rows are the contiguous range [2*q0, 2*q0 + 2*TPW) of patch p
-  const int nrp = row_pitch(nr);
-  float2* dst = out + (size_t)p * NBN * nrp + 2 * q0;
-  const int nrow = min(2 * TPW, nr - 2 * q0);
-  if constexpr (2 * TPW <= WG) {
-    const int rl = tid & (2 * TPW - 1);
-    if (rl < nrow)
-      for (int c = tid / (2 * TPW); c < NBN; c += WG / (2 * TPW)) dst[(size_t)c * nrp + rl] = sm[rl * NBN + c];
-  } else {
-    for (int e = tid; e < 2 * TPW * NBN; e += WG) {
-      const int c = e / (2 * TPW);
-      const int rl = e - c * (2 * TPW);
-      if (rl < nrow) dst[(size_t)c * nrp + rl] = sm[rl * NBN + c];
+    for (int mm = 0; mm <= 8; ++mm) {
+      const int c = t + TPT * mm;
+      if (c < NBN)
+        *(float4*)(dst + (size_t)c * nrp) = make_float4(za[mm].x, za[mm].y, zb[mm].x, zb[mm].y);
     }
   }
 }
