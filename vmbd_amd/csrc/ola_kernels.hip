@@ -37,6 +37,10 @@ struct cmax {
   static constexpr int v = A > B ? A : B;
 };
 
+// 16-byte global load of 4 consecutive floats at a 4-byte-aligned address
+// (gfx950 vector memory accepts unaligned addresses)
+__device__ __forceinline__ void ld4(float (&d)[4], const float* p) { __builtin_memcpy(d, p, 16); }
+
 // ----------------------------------------------------------------------------
 // rows_r2c
 // ----------------------------------------------------------------------------
@@ -109,24 +113,60 @@ __global__ __launch_bounds__(WG) void k_rows_r2c(Geo g, RowJobs jobs, const floa
   const float* row0 = base + (size_t)(ok0 ? i0 : 0) * s.ld;
   const float* row1 = base + (size_t)(ok1 ? i1 : 0) * s.ld;
   const float* w0 = wb ? wb + (size_t)r0 * g.sw1 : nullptr;
-  float2 v[16];
-#pragma unroll
-  for (int rr = 0; rr < 16; ++rr) {
-    const int c = t + TPT * rr;
-    const int j = o1 + c;
-    const bool okc = c >= s.clo && c < s.chi && j >= 0 && j < s.srcW;
-    float a = (ok0 && okc) ? row0[j] : 0.f;
-    float b = (ok1 && okc) ? row1[j] : 0.f;
-    if (wb) {
-      a = (ok0 && okc) ? a * w0[c] : 0.f;
-      b = (ok1 && okc) ? b * w0[g.sw1 + c] : 0.f;
-    }
-    v[rr] = make_float2(a, b);
-  }
 
   float2* sl = sm + tr * SL;
   float2 za[9], zb[9];
   if (wact) {
+    // Thread t reads columns 4m..4m+3, m = t + TPT*i, of both rows (and of
+    // their window rows) with 16-byte loads, all issued before any is used;
+    // chunks that cross the column range or the source edge take element
+    // loads.  The transform's LDS slice then hands column t + TPT*r to slot r
+    // of the FFT input mapping.
+    float a[4][4], b[4][4], wa[4][4], wb4[4][4];
+    bool full[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int c0 = 4 * (t + TPT * i);
+      const int j0 = o1 + c0;
+      full[i] = c0 >= s.clo && c0 + 4 <= s.chi && j0 >= 0 && j0 + 4 <= s.srcW;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) a[i][e] = b[i][e] = wa[i][e] = wb4[i][e] = 0.f;
+      if (full[i] && ok0) ld4(a[i], row0 + j0);
+      if (full[i] && ok1) ld4(b[i], row1 + j0);
+      if (wb && full[i] && ok0) ld4(wa[i], w0 + c0);
+      if (wb && full[i] && ok1) ld4(wb4[i], w0 + g.sw1 + c0);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int c0 = 4 * (t + TPT * i);
+      if (!full[i] && c0 + 4 > s.clo && c0 < s.chi) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int c = c0 + e;
+          const int j = o1 + c;
+          const bool okc = c >= s.clo && c < s.chi && j >= 0 && j < s.srcW;
+          if (ok0 && okc) a[i][e] = row0[j];
+          if (ok1 && okc) b[i][e] = row1[j];
+          if (wb && ok0 && okc) wa[i][e] = w0[c];
+          if (wb && ok1 && okc) wb4[i][e] = w0[g.sw1 + c];
+        }
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int c0 = 4 * (t + TPT * i);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const float x0 = wb ? a[i][e] * wa[i][e] : a[i][e];
+        const float x1 = wb ? b[i][e] * wb4[i][e] : b[i][e];
+        sl[lds_pad(c0 + e)] = make_float2(x0, x1);
+      }
+    }
+    wave_sync();
+    float2 v[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) v[r] = sl[lds_pad(t + TPT * r)];
+    wave_sync();
     fft<TW, -1>(v, sl, t, T, tw);
     // natural-order spectrum of z = a + i b into the slice, then split
 #pragma unroll
