@@ -386,32 +386,43 @@ hipError_t launch_cols(const Geo& g, int op, const ColIn& a, const ColIn& b, con
 // ----------------------------------------------------------------------------
 // inverse real row transform helpers
 // ----------------------------------------------------------------------------
-// Z[c] = A[c] + i B[c] over the full length from two half spectra (Hermitian
-// extension), in the input mapping of the transform.
+// Bins [0, TW/2] of each row are read with 16-byte loads of two bins (chunk k
+// holds bins 2k, 2k+1; thread t reads chunks t + TPT*i), all issued before
+// any is used.  Rows are stored with a pitch of at least TW/2 + 2 bins, so the
+// last chunk stays inside its row; its second bin is not used.  The full
+// spectrum Z[c] = A[c] + i B[c] (conjugate-symmetric halves mirrored to
+// TW - c) is written into the transform's LDS slice and slot r takes
+// Z[t + TPT*r].  A missing row (hasa / hasb false) reads as zero.
 template <int TW>
-__device__ __forceinline__ void c2r_load(float2 (&v)[16], const float2* ra, const float2* rb, bool hasb, int t) {
-  // ra, rb: valid rows (rb == ra when there is no second row, hasb false)
+__device__ __forceinline__ void c2r_load(float2 (&v)[16], const float2* ra, const float2* rb, bool hasa, bool hasb,
+                                         float2* sl, int t) {
   constexpr int TPT = TW / 16;
-  const float sb = hasb ? 1.f : 0.f;
+  constexpr int NCH = TW / 4 + 1;
+  constexpr int NI = (NCH + TPT - 1) / TPT;
+  float4 A[NI], B[NI];
 #pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    const int c = t + TPT * r;
-    if (TPT * r + TPT - 1 <= TW / 2) {  // whole slot in the stored half (folded after unrolling)
-      const float2 A = ra[c];
-      const float2 B = cscale(rb[c], sb);
-      v[r] = make_float2(A.x - B.y, A.y + B.x);
-    } else if (TPT * r > TW / 2) {      // whole slot mirrored
-      const float2 A = ra[TW - c];
-      const float2 B = cscale(rb[TW - c], sb);
-      v[r] = make_float2(A.x + B.y, B.x - A.y);
-    } else {
-      const bool lo = c <= TW / 2;
-      const int cc = lo ? c : TW - c;
-      const float2 A = ra[cc];
-      const float2 B = cscale(rb[cc], sb);
-      v[r] = lo ? make_float2(A.x - B.y, A.y + B.x) : make_float2(A.x + B.y, B.x - A.y);
+  for (int i = 0; i < NI; ++i) {
+    const int k = t + TPT * i;
+    A[i] = B[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (hasa && k < NCH) __builtin_memcpy(&A[i], ra + 2 * k, 16);
+    if (hasb && k < NCH) __builtin_memcpy(&B[i], rb + 2 * k, 16);
+  }
+#pragma unroll
+  for (int i = 0; i < NI; ++i) {
+    const int k = t + TPT * i;
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      const int c = 2 * k + e;
+      const float2 a = e ? make_float2(A[i].z, A[i].w) : make_float2(A[i].x, A[i].y);
+      const float2 b = e ? make_float2(B[i].z, B[i].w) : make_float2(B[i].x, B[i].y);
+      if (c <= TW / 2) sl[lds_pad(c)] = make_float2(a.x - b.y, a.y + b.x);
+      if (c > 0 && c < TW / 2) sl[lds_pad(TW - c)] = make_float2(a.x + b.y, b.x - a.y);
     }
   }
+  wave_sync();
+#pragma unroll
+  for (int r = 0; r < 16; ++r) v[r] = sl[lds_pad(t + TPT * r)];
+  wave_sync();
 }
 
 // ----------------------------------------------------------------------------
@@ -487,10 +498,10 @@ __global__ __launch_bounds__(WG) void k_rows_c2r_ola(Geo g, OlaDst d, const floa
     const float* wa = wrow ? wl + (mi * d.nslot + d.win_slot[ka - kf]) * g.sw1 : wl;
     const float* wbp = (wrow && hasb) ? wl + (mi * d.nslot + d.win_slot[ka + 1 - kf]) * g.sw1 : wa;
     float2 v[16];
-    // a wavefront with no item in this round skips its transforms
-    const bool wact = base + (int)(threadIdx.x >> 6) * (64 / TPT) < NT;
-    if (wact) {
-      c2r_load<TW>(v, ra, rb, rok && hasb, t);
+    // transforms without an item skip their work and never touch LDS, so a
+    // single-round launch holds slices for its items only
+    if (act) {
+      c2r_load<TW>(v, ra, rb, rok, rok && hasb, sl, t);
       fft<TW, +1>(v, sl, t, T, tw);
     } else {
 #pragma unroll
@@ -549,7 +560,8 @@ __global__ __launch_bounds__(WG) void k_rows_c2r_ola(Geo g, OlaDst d, const floa
 __host__ __device__ int ola_window_offset_floats(const Geo& g, const OlaDst& d) {
   const int TPT = g.Tw / 16;
   const int TPW = WG / TPT;
-  const int fl = TPW * (g.Tw + g.Tw / 16) * 2;
+  const int nsl = d.alias ? d.mc * ((g.csf1 + 1) / 2) : TPW;  // FFT slices in use
+  const int fl = nsl * (g.Tw + g.Tw / 16) * 2;
   const int rb = d.mc * g.csf1 * d.ext1;
   return d.alias ? (fl > rb ? fl : rb) : fl + rb;
 }
@@ -617,8 +629,8 @@ __global__ __launch_bounds__(WG) void k_rows_c2r_stack(Geo g, const float2* __re
   Twiddles<TW> T;
   twiddles_init<TW>(T, t, tw);
   float2 v[16];
-  c2r_load<TW>(v, ra, rb, hasb, t);
   float2* sl = fl + tr * SL;
+  c2r_load<TW>(v, ra, rb, act, hasb, sl, t);
   fft<TW, +1>(v, sl, t, T, tw);
   if (act) {
 #pragma unroll
