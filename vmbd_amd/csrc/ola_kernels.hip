@@ -394,21 +394,33 @@ hipError_t launch_cols(const Geo& g, int op, const ColIn& a, const ColIn& b, con
 // TW - c) is written into the transform's LDS slice and slot r takes
 // Z[t + TPT*r].  A missing row (hasa / hasb false) reads as zero.
 template <int TW>
-__device__ __forceinline__ void c2r_load(float2 (&v)[16], const float2* ra, const float2* rb, bool hasa, bool hasb,
-                                         float2* sl, int t) {
-  constexpr int TPT = TW / 16;
-  constexpr int NCH = TW / 4 + 1;
-  constexpr int NI = (NCH + TPT - 1) / TPT;
-  float4 A[NI], B[NI];
+struct C2r {
+  static constexpr int TPT = TW / 16;
+  static constexpr int NCH = TW / 4 + 1;                // two-bin chunks covering [0, TW/2 + 1]
+  static constexpr int NI = (NCH + TPT - 1) / TPT;      // chunks per thread
+  static_assert(NI == 5, "chunk count per thread");
+};
+
+// global loads of both rows' chunks (all issued before any is used)
+template <int TW>
+__device__ __forceinline__ void c2r_fetch(float4 (&A)[5], float4 (&B)[5], const float2* ra, const float2* rb,
+                                          bool hasa, bool hasb, int t) {
 #pragma unroll
-  for (int i = 0; i < NI; ++i) {
-    const int k = t + TPT * i;
+  for (int i = 0; i < 5; ++i) {
+    const int k = t + C2r<TW>::TPT * i;
     A[i] = B[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (hasa && k < NCH) __builtin_memcpy(&A[i], ra + 2 * k, 16);
-    if (hasb && k < NCH) __builtin_memcpy(&B[i], rb + 2 * k, 16);
+    if (hasa && k < C2r<TW>::NCH) __builtin_memcpy(&A[i], ra + 2 * k, 16);
+    if (hasb && k < C2r<TW>::NCH) __builtin_memcpy(&B[i], rb + 2 * k, 16);
   }
+}
+
+// Hermitian extension in the slice and read-back in the input mapping
+template <int TW>
+__device__ __forceinline__ void c2r_stage(float2 (&v)[16], const float4 (&A)[5], const float4 (&B)[5], float2* sl,
+                                          int t) {
+  constexpr int TPT = C2r<TW>::TPT;
 #pragma unroll
-  for (int i = 0; i < NI; ++i) {
+  for (int i = 0; i < 5; ++i) {
     const int k = t + TPT * i;
 #pragma unroll
     for (int e = 0; e < 2; ++e) {
@@ -425,10 +437,59 @@ __device__ __forceinline__ void c2r_load(float2 (&v)[16], const float2* ra, cons
   wave_sync();
 }
 
+template <int TW>
+__device__ __forceinline__ void c2r_load(float2 (&v)[16], const float2* ra, const float2* rb, bool hasa, bool hasb,
+                                         float2* sl, int t) {
+  float4 A[5], B[5];
+  c2r_fetch<TW>(A, B, ra, rb, hasa, hasb, t);
+  c2r_stage<TW>(v, A, B, sl, t);
+}
+
 // ----------------------------------------------------------------------------
 // rows_c2r_ola: one workgroup per output row
 // ----------------------------------------------------------------------------
 __host__ __device__ int ola_window_offset_floats(const Geo& g, const OlaDst& d);
+
+// Window rows of output row u: row u - m*hop0 of every window class of every
+// covering patch row m (pair pr = mi*nslot + slot), zero where the row lies
+// outside the window or the slot is unused.  Each wavefront takes pairs
+// pr = w + 4k; a lane loads columns lane + 64j of two pairs before writing
+// any of them.
+template <int TW>
+__device__ __forceinline__ void ola_stage_windows(float* wl, const Geo& g, const OlaDst& d, int Mc, int mlo, int u) {
+  constexpr int NW = WG / 64;
+  constexpr int JU = (TW + 63) / 64;  // sw1 <= TW
+  const int npair = Mc * d.nslot;
+  const int lane = threadIdx.x & 63;
+  for (int p0 = (int)(threadIdx.x >> 6); p0 < npair; p0 += 2 * NW) {
+    float w[2][JU];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const int pr = p0 + k * NW;
+      const int mi = pr / d.nslot;
+      const int slot = pr - mi * d.nslot;
+      const int m = mlo + mi;
+      const int r = u - m * g.hop0;
+      const int cls = pr < npair ? d.row_class[m * d.nslot + slot] : -1;
+      const bool ok = cls >= 0 && r >= 0 && r < g.sw0;
+      const float* src = d.win + ((size_t)(ok ? cls : 0) * g.sw0 + (ok ? r : 0)) * g.sw1;
+#pragma unroll
+      for (int j = 0; j < JU; ++j) {
+        const int c = lane + 64 * j;
+        w[k][j] = (ok && c < g.sw1) ? src[c] : 0.f;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const int pr = p0 + k * NW;
+#pragma unroll
+      for (int j = 0; j < JU; ++j) {
+        const int c = lane + 64 * j;
+        if (pr < npair && c < g.sw1) wl[pr * g.sw1 + c] = w[k][j];
+      }
+    }
+  }
+}
 
 
 template <int TW>
@@ -466,17 +527,7 @@ __global__ __launch_bounds__(WG) void k_rows_c2r_ola(Geo g, OlaDst d, const floa
   // row, window class of that row), staged in LDS past the gather buffer
   float* wl = (float*)(dyn) + ola_window_offset_floats(g, d);
   if (d.win) {
-    const int npair = Mc * d.nslot;
-    for (int pr = (int)(threadIdx.x >> 6); pr < npair; pr += WG / 64) {
-      const int mi = pr / d.nslot;
-      const int sl = pr - mi * d.nslot;
-      const int m = mlo + mi;
-      const int r = u - m * g.hop0;
-      const int cls = d.row_class[m * d.nslot + sl];
-      const bool ok = cls >= 0 && r >= 0 && r < g.sw0;
-      const float* src = ok ? d.win + ((size_t)cls * g.sw0 + r) * g.sw1 : d.win;
-      for (int c = threadIdx.x & 63; c < g.sw1; c += 64) wl[pr * g.sw1 + c] = ok ? src[c] : 0.f;
-    }
+    ola_stage_windows<TW>(wl, g, d, Mc, mlo, u);
     __syncthreads();
   }
 
