@@ -263,8 +263,9 @@ __device__ __forceinline__ void col_load(float2 (&v)[16], const ColIn& in, int p
 }
 
 // stage rows [0,nrows) of the strip in LDS and store row-major [P][nrows][NBP].
-// The staging area aliases the FFT slices: barriers on both sides.
-template <int TH, int CS>
+// The staging area aliases the FFT slices: barriers on both sides, except
+// after the kernel's last store (LAST), where no slice is used again.
+template <int TH, int CS, bool LAST = false>
 __device__ __forceinline__ void col_store_rows(const float2 (&v)[16], float2* st, const ColOut& o, int p, int strip,
                                                int NB, int NBP, int cc, int t) {
   constexpr int TPT = TH / 16;
@@ -281,7 +282,7 @@ __device__ __forceinline__ void col_store_rows(const float2 (&v)[16], float2* st
     float2* dst = o.p + (size_t)p * o.nrows * NBP + c0 + ci;
     for (int r = threadIdx.x / CS; r < o.nrows; r += WG / CS) dst[(size_t)r * NBP] = st[r * (CS + 1) + ci];
   }
-  __syncthreads();  // staging drained before the slices are reused
+  if (!LAST) __syncthreads();  // staging drained before the slices are reused
 }
 
 template <int TH, int OP>
@@ -331,14 +332,14 @@ __global__ __launch_bounds__(WG) void k_cols(Geo g, ColIn a, ColIn b, ColIn c3, 
     for (int m = 0; m < 16; ++m)
       B[m] = cscale((OP == COL_FWD) ? cmul(A[m], B[m]) : cmulc(A[m], B[m]), scale);
     if (wact) fft<TH, +1>(B, sl, t, T, tw);
-    col_store_rows<TH, CS>(B, sm, o1, p, strip, NB, g.NBP, cc, t);
+    col_store_rows<TH, CS, OP != COL_ADJ2>(B, sm, o1, p, strip, NB, g.NBP, cc, t);
     if constexpr (OP == COL_ADJ2) {
       col_load<TH>(B, c3, p, c, NB, t, active, grp, row0);
       if (!c3.spec && wact) fft<TH, -1>(B, sl, t, T, tw);
 #pragma unroll
       for (int m = 0; m < 16; ++m) B[m] = cscale(cmulc(A[m], B[m]), scale);
       if (wact) fft<TH, +1>(B, sl, t, T, tw);
-      col_store_rows<TH, CS>(B, sm, o2, p, strip, NB, g.NBP, cc, t);
+      col_store_rows<TH, CS, true>(B, sm, o2, p, strip, NB, g.NBP, cc, t);
     }
   }
 }
