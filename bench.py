@@ -40,7 +40,7 @@ def parse():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=50)
     ap.add_argument('--warmup', type=int, default=5)
-    ap.add_argument('--frames', type=int, default=8, help='frames per step, batched into the same launches')
+    ap.add_argument('--frames', type=int, default=32, help='frames per step, batched into the same launches')
     ap.add_argument('--no-graph', action='store_true', help='launch eagerly instead of replaying a HIP graph')
     ap.add_argument('--cpu-seconds', type=float, default=12.0, help='bounded CPU-baseline sample')
     ap.add_argument('--no-cpu', action='store_true')
@@ -97,21 +97,25 @@ def operator_bytes():
     return 4.0 * 3 * (Sx + Sy + Pf)
 
 
-TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'profiles', 'ola_c3_b8_hbm_traffic.md')
+TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'profiles',
+                            'ola_1024_10x10_sf41_b32_hbm_traffic.md')
 
 
-def measured_traffic(kernel):
+def measured_traffic(kernel, frames):
     """HBM bytes per launch of `kernel` at this workload from the committed
     rocprofv3 --pmc summary (FETCH_SIZE x its calibration factor +
-    WRITE_SIZE), or None when the summary has no line for it."""
+    WRITE_SIZE), or None when the summary has no line for it or was measured
+    at another number of frames per launch."""
     try:
         with open(TRAFFIC_FILE) as fh:
-            for line in fh:
-                parts = line.split()
-                if len(parts) == 3 and parts[0] == 'traffic_bytes_per_launch' and parts[1] == kernel:
-                    return int(parts[2])
+            lines = [ln.split() for ln in fh]
     except OSError:
-        pass
+        return None
+    if ['frames_per_launch', str(frames)] not in lines:
+        return None
+    for parts in lines:
+        if len(parts) == 3 and parts[0] == 'traffic_bytes_per_launch' and parts[1] == kernel:
+            return int(parts[2])
     return None
 
 
@@ -307,7 +311,7 @@ def main():
     d = kernels[dom]
     roofline = dict(kernel=dom, bound='hbm', achieved=d.get('hbm_gbs', 0.0), peak=HBM_PEAK_GBS, unit='GB/s',
                     frac=round(d.get('hbm_gbs', 0.0) / HBM_PEAK_GBS, 4),
-                    traffic=measured_traffic(dom) if B == 8 else None, avg_us=d['avg_us'],
+                    traffic=measured_traffic(dom, B), avg_us=d['avg_us'],
                     valu=dict(achieved_tflops=d.get('tflops', 0.0), peak_nonpacked=VALU_FP32_PEAK,
                               frac=round(d.get('tflops', 0.0) / VALU_FP32_PEAK, 4)))
     op_bytes = operator_bytes() * B
