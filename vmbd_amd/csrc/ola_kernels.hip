@@ -643,6 +643,7 @@ static hipError_t c2r_ola_t(const Geo& g, const OlaDst& d, const float2* tw, hip
 
 hipError_t launch_rows_c2r_ola(const Geo& g, const OlaDst& d, const float2* tw, hipStream_t st) {
   if (d.O0 <= 0 || d.O1 <= 0) return hipSuccess;
+  if (g.NBP < g.Tw / 2 + 2) return hipErrorInvalidValue;  // two-bin loads need a padding bin per row
   switch (g.Tw) {
     case 16: return c2r_ola_t<16>(g, d, tw, st);
     case 32: return c2r_ola_t<32>(g, d, tw, st);
@@ -714,6 +715,7 @@ static hipError_t c2r_stack_t(const Geo& g, const float2* src, int srow, float* 
 
 hipError_t launch_rows_c2r_stack(const Geo& g, const float2* src, int srow, float* out, int nrows, int ncols,
                                  const float2* tw, hipStream_t st, const float* axz, float beta) {
+  if (g.NBP < g.Tw / 2 + 2) return hipErrorInvalidValue;  // two-bin loads need a padding bin per row
   switch (g.Tw) {
     case 16: return c2r_stack_t<16>(g, src, srow, out, nrows, ncols, tw, st, axz, beta);
     case 32: return c2r_stack_t<32>(g, src, srow, out, nrows, ncols, tw, st, axz, beta);
