@@ -50,7 +50,7 @@ __global__ __launch_bounds__(WG) void k_rows_r2c(Geo g, RowJobs jobs, const floa
   constexpr int TPW = WG / TPT;
   constexpr int NBN = TW / 2 + 1;
   constexpr int SL = FftLds<TW>::floats2;
-  __shared__ float2 sm[TPW * SL];
+  __shared__ float2 sm[cmax<TPW * SL, 2 * TPW * NBN>::v];
 
   // 1-D grid over work units (patch, row block), row blocks of all jobs
   // job-major.  Workgroup ids are dealt round-robin to the NXCD XCDs; XCD j
@@ -185,21 +185,30 @@ __global__ __launch_bounds__(WG) void k_rows_r2c(Geo g, RowJobs jobs, const floa
 #pragma unroll
     for (int mm = 0; mm <= 8; ++mm) za[mm] = zb[mm] = make_float2(0.f, 0.f);
   }
-  // bin-major store without staging: bin c of rows 2q (A) and 2q+1 (B) is
-  // one 16-byte write; the four row pairs of a wavefront cover 64 contiguous
-  // bytes of each bin and the workgroup's 32 rows (row blocks start on
-  // 32-row, 256-byte boundaries of the 16-row pitch) two whole lines, which
-  // fill in L2 before they are written back
-  if (active) {
-    const int nrp = row_pitch(nr);
-    float2* dst = out + (size_t)p * NBN * nrp + 2 * q;
+  // bin-major store staged through LDS: lane pairs of rows (2rp, 2rp+1) of
+  // one bin are one 16-byte write, so the workgroup's 32 rows of a bin (row
+  // blocks start on 32-row, 256-byte boundaries of the 16-row pitch) are two
+  // whole lines written by 16 consecutive lanes
+  __syncthreads();  // every slice has been read: the buffer becomes staging
 #pragma unroll
-    for (int mm = 0; mm <= 8; ++mm) {
-      const int c = t + TPT * mm;
-      if (c < NBN)
-        *(float4*)(dst + (size_t)c * nrp) = make_float4(za[mm].x, za[mm].y, zb[mm].x, zb[mm].y);
+  for (int mm = 0; mm <= 8; ++mm) {
+    const int c = t + TPT * mm;
+    if (c < NBN) {
+      sm[(2 * tr) * NBN + c] = za[mm];
+      sm[(2 * tr + 1) * NBN + c] = zb[mm];
     }
   }
+  __syncthreads();
+  const int nrp = row_pitch(nr);
+  float2* dst = out + (size_t)p * NBN * nrp + 2 * q0;
+  const int nrow = min(2 * TPW, nr - 2 * q0);
+  // (an odd row count writes its last pair into the padding row of the pitch)
+  const int rp = tid & (TPW - 1);
+  if (2 * rp < nrow)
+    for (int c = tid / TPW; c < NBN; c += WG / TPW) {
+      const float2 a = sm[(2 * rp) * NBN + c], b = sm[(2 * rp + 1) * NBN + c];
+      *(float4*)(dst + (size_t)c * nrp + 2 * rp) = make_float4(a.x, a.y, b.x, b.y);
+    }
 }
 
 template <int TW>
