@@ -4,11 +4,12 @@
 // convolution on a Th x Tw tile, split in three kernels so that every launch
 // has hundreds to thousands of workgroups and all global traffic is coalesced:
 //
-//   rows_r2c      chop + window + zero pad fused into the load, real row FFTs
-//                 (two rows packed in one complex transform), half spectrum
-//                 written bin-major [P][NB][row_pitch(rows)] so the column
-//                 pass reads contiguous runs.  Rows that are all zero are
-//                 never computed.
+//   rows_r2c      chop + window + zero pad fused into the load (16-byte
+//                 loads, redistributed to the FFT mapping through LDS), real
+//                 row FFTs (two rows packed in one complex transform), half
+//                 spectrum written bin-major [P][NB][row_pitch(rows)] so the
+//                 column pass reads contiguous runs.  Rows that are all zero
+//                 are never computed.
 //   cols          per (patch, strip of bins): column FFTs of 1..3 inputs, the
 //                 spectrum product (conj for the transposes), inverse column
 //                 FFT; only the output rows that are needed are stored,
@@ -22,7 +23,7 @@
 //
 // LDS: each workgroup owns one buffer; the FFT slices and the staging area of
 // the transposed store alias it (separated by workgroup barriers), which keeps
-// the footprint near 35 KB and allows four workgroups per CU.
+// the footprint at 35-40 KB and allows four workgroups per CU.
 #include "fft_core.h"
 #include "ola_kernels.h"
 #include "prof.h"
