@@ -65,6 +65,26 @@ def test_forward_and_adjoints(geom, mode, cuda):
     torch.cuda.synchronize()
 
 
+@pytest.mark.parametrize('geom', [((200, 400), (4, 24), (9, 9), 0.75), ((48, 2200), (1, 80), (7, 7), 0.75)],
+                         ids=['streamed-rounds', 'wide-rows'])
+def test_multi_round_ola(geom, cuda):
+    """Output rows covered by more patch pairs than one workgroup round holds
+    (60 and 40 pairs for 32 slots): the OLA pass runs several rounds per row,
+    gathering each round into LDS column accumulators (rows up to 2048 wide,
+    first case) or all rounds at once (the 2194-wide rows of the second).
+    Forward and exact adjoints against the oracle."""
+    sx, csf, sf, ov = geom
+    print(f'case multi-round sx={sx} csf={csf} sf={sf} ov={ov}', flush=True)
+    wa, plan, owa = _mk(sx, csf, ov, sf, 'valid', cuda)
+    x, f, g = case_inputs(12, sx, csf, sf)
+    x32, f32, g32 = (np.float32(a).astype(np.float64) for a in (x, f, g))
+    xt, ft, gt = _t(x, cuda), _t(f, cuda), _t(g, cuda)
+    assert rel_l2(plan.forward(xt, ft).cpu().numpy(), O.forward(x32, f32, owa)) <= TOL
+    xb, fb = plan.adjoint_xf(gt, xt, ft, exact=True)
+    assert rel_l2(xb.cpu().numpy(), O.adjoint_x(g32, f32, owa, sx, exact=True)) <= TOL
+    assert rel_l2(fb.cpu().numpy(), O.adjoint_f(g32, x32, owa, sf, exact=True)) <= TOL
+
+
 @pytest.mark.parametrize('geom', GEOMS[:4], ids=lambda g: f'{g[0]}-{g[1]}-{g[2]}')
 def test_cached_spectra(geom, cuda):
     """OlaGPU semantics: F instance caches F^, X instance caches X^."""

@@ -202,6 +202,7 @@ OlaDst dst_forward(const vmbd_plan* p, float* y) {
   d.win_idx = nullptr;
   d.mc = p->mc_fwd;
   d.alias = ola_single_round(g, d);
+  d.stream = ola_stream_rounds(g, d);
   return d;
 }
 
@@ -224,6 +225,7 @@ OlaDst dst_adjoint_x(const vmbd_plan* p, float* xbar) {
   d.nslot = p->nslot;
   d.mc = p->mc_adj;
   d.alias = ola_single_round(g, d);
+  d.stream = ola_stream_rounds(g, d);
   return d;
 }
 

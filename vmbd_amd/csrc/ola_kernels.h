@@ -95,7 +95,9 @@ struct OlaDst {
   const int* row_class;
   int nslot;
   int mc;     // max number of patch rows covering one output row
-  int alias;  // 1: all transforms of a row fit one round; gather buffer aliases the FFT slices
+  int alias;   // 1: all transforms of a row fit one round; gather buffer aliases the FFT slices
+  int stream;  // 1: several rounds, each round's rows (aliasing the FFT slices) gathered into
+               //    per-thread column accumulators; needs O1 <= ola_stream_cols()
   // fused solver arithmetic on the stored value v (each optional):
   // v = ((v - sub[o]) * mul[o]) + beta * axz[o], o = output index
   const float* sub;
@@ -120,5 +122,7 @@ hipError_t launch_rows_c2r_stack(const Geo& g, const float2* src, int srow, floa
 size_t ola_rowbuf_bytes(const Geo& g, const OlaDst& d);
 size_t ola_shared_bytes(const Geo& g, const OlaDst& d);
 int ola_single_round(const Geo& g, const OlaDst& d);
+// multi-round launches whose output rows fit the per-thread accumulators
+int ola_stream_rounds(const Geo& g, const OlaDst& d);
 
 }  // namespace vmbd

@@ -503,15 +503,22 @@ __device__ __forceinline__ void ola_stage_windows(float* wl, const Geo& g, const
 }
 
 
-template <int TW>
+constexpr int OLA_KJ = 8;  // output columns per thread of streamed launches (O1 <= OLA_KJ * WG)
+
+// STREAM: several rounds of TPW transforms; each round's rows go to a buffer
+// that aliases the FFT slices and are added at once into per-column
+// accumulators in LDS (fixed round and patch order), so the LDS holds one
+// round instead of every covering patch row.  Otherwise all rows of the
+// output row are gathered after the last round (single-round launches alias
+// that buffer with the slices).
+template <int TW, bool STREAM>
 __global__ __launch_bounds__(WG) void k_rows_c2r_ola(Geo g, OlaDst d, const float2* __restrict__ tw) {
   constexpr int TPT = TW / 16;
   constexpr int TPW = WG / TPT;
   constexpr int SL = FftLds<TW>::floats2;
   extern __shared__ float2 dyn[];
   float2* fl = dyn;
-  // single-round launches alias the gather buffer with the FFT slices
-  float* rowbuf = d.alias ? (float*)dyn : (float*)(dyn + TPW * SL);
+  float* rowbuf = (STREAM || d.alias) ? (float*)dyn : (float*)(dyn + TPW * SL);
 
   const int fr = blockIdx.y;
   const int i = blockIdx.x;
@@ -542,6 +549,13 @@ __global__ __launch_bounds__(WG) void k_rows_c2r_ola(Geo g, OlaDst d, const floa
     __syncthreads();
   }
 
+  const float inv_hop1 = 1.0f / (float)g.hop1;
+  // streamed launches: this row's accumulators in LDS past the staged
+  // windows; column j belongs to thread j mod WG, so no barrier guards them
+  float* accl = wl + (d.win ? d.mc * d.nslot * g.sw1 : 0);
+  if constexpr (STREAM)
+    for (int j = threadIdx.x; j < d.O1; j += WG) accl[j] = 0.f;
+
   for (int base = 0; base < NT; base += TPW) {
     const int tix = base + tr;
     const bool act = tix < NT;
@@ -569,10 +583,10 @@ __global__ __launch_bounds__(WG) void k_rows_c2r_ola(Geo g, OlaDst d, const floa
 #pragma unroll
       for (int mm = 0; mm < 16; ++mm) v[mm] = make_float2(0.f, 0.f);
     }
-    if (d.alias) __syncthreads();  // slices are about to be overwritten by rowbuf
+    if (STREAM || d.alias) __syncthreads();  // slices are about to be overwritten by rowbuf
     if (act) {
-      float* ba = rowbuf + (mi * g.csf1 + na) * d.ext1;
-      float* bb = rowbuf + (mi * g.csf1 + nb) * d.ext1;
+      float* ba = rowbuf + (STREAM ? 2 * tr : mi * g.csf1 + na) * d.ext1;
+      float* bb = rowbuf + (STREAM ? 2 * tr + 1 : mi * g.csf1 + nb) * d.ext1;
 #pragma unroll
       for (int mm = 0; mm < 16; ++mm) {
         const int col = t + TPT * mm;
@@ -588,10 +602,43 @@ __global__ __launch_bounds__(WG) void k_rows_c2r_ola(Geo g, OlaDst d, const floa
         }
       }
     }
+    if constexpr (STREAM) {
+      __syncthreads();
+      // this round's items tix in [tlo, thi): patch row mi, pair n/2
+      const int tlo = base, thi = min(base + TPW, NT);
+      for (int j = threadIdx.x; j < d.O1; j += WG) {
+        const int vcol = j + d.oc1;
+        if (vcol >= 0) {
+          int qh = (int)((float)vcol * inv_hop1);
+          qh += ((qh + 1) * g.hop1 <= vcol) - (qh * g.hop1 > vcol);
+          const int nhi = min(g.csf1 - 1, qh);
+          const int lc0 = vcol - nhi * g.hop1;
+          for (int mr = tlo / NP2; mr <= (thi - 1) / NP2; ++mr) {
+            float sub = 0.f;
+            for (int nn = nhi, lc = lc0; nn >= 0 && lc < d.ext1; --nn, lc += g.hop1) {
+              const int tx = mr * NP2 + (nn >> 1);
+              if (tx >= tlo && tx < thi) sub += rowbuf[(2 * (tx - tlo) + (nn & 1)) * d.ext1 + lc];
+            }
+            accl[j] += sub;
+          }
+        }
+      }
+      __syncthreads();  // the round's rows are read before the next round's transforms
+    }
+  }
+  if constexpr (STREAM) {
+    for (int j = threadIdx.x; j < d.O1; j += WG) {
+      const size_t o = ((size_t)fr * d.O0 + i) * d.O1 + j;
+      float a = accl[j];
+      if (d.sub) a -= d.sub[o];
+      if (d.mul) a *= d.mul[o];
+      if (d.axz) a += d.beta * d.axz[o];
+      d.out[o] = a;
+    }
+    return;
   }
   __syncthreads();
   // fixed-order gather over covering patches
-  const float inv_hop1 = 1.0f / (float)g.hop1;
   for (int j = threadIdx.x; j < d.O1; j += WG) {
     const int vcol = j + d.oc1;
     float acc = 0.f;
@@ -624,17 +671,19 @@ __host__ __device__ int ola_window_offset_floats(const Geo& g, const OlaDst& d) 
   const int TPW = WG / TPT;
   const int nsl = d.alias ? d.mc * ((g.csf1 + 1) / 2) : TPW;  // FFT slices in use
   const int fl = nsl * (g.Tw + g.Tw / 16) * 2;
-  const int rb = d.mc * g.csf1 * d.ext1;
-  return d.alias ? (fl > rb ? fl : rb) : fl + rb;
+  const int rb = d.stream ? 2 * TPW * d.ext1 : d.mc * g.csf1 * d.ext1;
+  return (d.alias || d.stream) ? (fl > rb ? fl : rb) : fl + rb;
 }
 
 size_t ola_rowbuf_bytes(const Geo& g, const OlaDst& d) {
-  return (size_t)d.mc * g.csf1 * d.ext1 * sizeof(float);
+  const int TPW = WG / (g.Tw / 16);
+  return (size_t)(d.stream ? 2 * TPW : d.mc * g.csf1) * d.ext1 * sizeof(float);
 }
 
 size_t ola_shared_bytes(const Geo& g, const OlaDst& d) {
   const size_t wl = d.win ? (size_t)d.mc * d.nslot * g.sw1 * sizeof(float) : 0;
-  return (size_t)ola_window_offset_floats(g, d) * sizeof(float) + wl;
+  const size_t acc = d.stream ? (size_t)d.O1 * sizeof(float) : 0;
+  return (size_t)ola_window_offset_floats(g, d) * sizeof(float) + wl + acc;
 }
 
 int ola_single_round(const Geo& g, const OlaDst& d) {
@@ -642,12 +691,19 @@ int ola_single_round(const Geo& g, const OlaDst& d) {
   return d.mc * ((g.csf1 + 1) / 2) <= TPW;
 }
 
+int ola_stream_rounds(const Geo& g, const OlaDst& d) { return !ola_single_round(g, d) && d.O1 <= OLA_KJ * WG; }
+
 template <int TW>
 static hipError_t c2r_ola_t(const Geo& g, const OlaDst& d, const float2* tw, hipStream_t st) {
   const int frames = g.P / g.Pf;
   const size_t shm = ola_shared_bytes(g, d);
   ProfScope ps(d.win ? "rows_c2r_ola_win" : "rows_c2r_ola", st);
-  hipLaunchKernelGGL(k_rows_c2r_ola<TW>, dim3((unsigned)d.O0, (unsigned)frames), dim3(WG), shm, st, g, d, tw);
+  if (d.stream)
+    hipLaunchKernelGGL((k_rows_c2r_ola<TW, true>), dim3((unsigned)d.O0, (unsigned)frames), dim3(WG), shm, st, g, d,
+                       tw);
+  else
+    hipLaunchKernelGGL((k_rows_c2r_ola<TW, false>), dim3((unsigned)d.O0, (unsigned)frames), dim3(WG), shm, st, g, d,
+                       tw);
   return hipGetLastError();
 }
 
