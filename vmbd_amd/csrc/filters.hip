@@ -87,6 +87,45 @@ __global__ __launch_bounds__(BX* BY) void k_bfilter(float* out, const float* a, 
   out[(long long)i * W + j] = acc / sw;
 }
 
+// The same filter for a half size WR known at compile time (the MFBD prior's
+// w = 5): loops unrolled, the spatial factor folded into one exponent,
+// exp(-t) evaluated as the hardware 2^(-t log2 e), and the in-image test
+// only in workgroups whose window crosses the image border.  Same sums in
+// the same order as k_bfilter.
+template <int WR>
+__global__ __launch_bounds__(BX* BY) void k_bfilter_w(float* out, const float* a, int H, int W, float sd, float sr) {
+  extern __shared__ float tile[];
+  constexpr int TW = BX + 2 * WR, TH = BY + 2 * WR;
+  const int i0 = blockIdx.y * BY - WR, j0 = blockIdx.x * BX - WR;
+  for (int idx = threadIdx.y * BX + threadIdx.x; idx < TW * TH; idx += BX * BY) {
+    const int ti = idx / TW, tj = idx - (idx / TW) * TW;
+    const int ii = i0 + ti, jj = j0 + tj;
+    tile[idx] = (ii >= 0 && ii < H && jj >= 0 && jj < W) ? a[(long long)ii * W + jj] : 0.f;
+  }
+  __syncthreads();
+  const int i = blockIdx.y * BY + threadIdx.y, j = blockIdx.x * BX + threadIdx.x;
+  if (i >= H || j >= W) return;
+  const bool inner = i0 >= 0 && j0 >= 0 && i0 + TH <= H && j0 + TW <= W;  // uniform per workgroup
+  const float* tc = tile + (threadIdx.y + WR) * TW + threadIdx.x + WR;
+  const float c = tc[0];
+  constexpr float LOG2E = 1.44269504088896340736f;
+  const float kr = -LOG2E / (2.f * sr * sr), kd = -LOG2E / (2.f * sd * sd);
+  float sw = 0.f, acc = 0.f;
+#pragma unroll
+  for (int x = -WR; x <= WR; ++x) {
+#pragma unroll
+    for (int y = -WR; y <= WR; ++y) {
+      const float v = tc[y * TW + x];
+      const float dv = v - c;
+      float wt = __builtin_amdgcn_exp2f(fmaf(dv * dv, kr, (float)(x * x + y * y) * kd));
+      if (!inner && !(i + y >= 0 && i + y < H && j + x >= 0 && j + x < W)) wt = 0.f;
+      sw += wt;
+      acc += wt * v;
+    }
+  }
+  out[(long long)i * W + j] = acc / sw;
+}
+
 #define SQRT2H 0.70710678118654752440f
 
 // kernel.cu:180-242
@@ -253,7 +292,10 @@ int vmbd_bilateral(float* out, const float* in, int32_t H, int32_t W, int32_t w,
   hipStream_t st = (hipStream_t)stream;
   const size_t shm = (size_t)(BX + 2 * w) * (BY + 2 * w) * sizeof(float);
   ProfScope ps("bilateral", st);
-  hipLaunchKernelGGL(k_bfilter, grid2(H, W), dim3(BX, BY), shm, st, out, in, H, W, w, sigma_d, sigma_r);
+  if (w == 5)
+    hipLaunchKernelGGL(k_bfilter_w<5>, grid2(H, W), dim3(BX, BY), shm, st, out, in, H, W, sigma_d, sigma_r);
+  else
+    hipLaunchKernelGGL(k_bfilter, grid2(H, W), dim3(BX, BY), shm, st, out, in, H, W, w, sigma_d, sigma_r);
   FILT_DONE("vmbd_bilateral");
 }
 
