@@ -295,8 +295,10 @@ __device__ __forceinline__ void col_store_rows(const float2 (&v)[16], float2* st
   if (!LAST) __syncthreads();  // staging drained before the slices are reused
 }
 
+// 512-point columns: four wavefronts per SIMD (128 VGPRs, a few spilled)
+// beat three without spills
 template <int TH, int OP>
-__global__ __launch_bounds__(WG) void k_cols(Geo g, ColIn a, ColIn b, ColIn c3, ColOut o1, ColOut o2, float scale,
+__global__ __launch_bounds__(WG, (TH >= 512 ? 4 : 1)) void k_cols(Geo g, ColIn a, ColIn b, ColIn c3, ColOut o1, ColOut o2, float scale,
                                              float2* __restrict__ spec_out, const float2* __restrict__ tw) {
   constexpr int TPT = TH / 16;
   constexpr int CS = WG / TPT;  // columns per workgroup
