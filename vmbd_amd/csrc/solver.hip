@@ -8,6 +8,9 @@
 // shuffles (DPP/permute through __shfl_xor) and one LDS exchange per wave.
 #include <hip/hip_runtime.h>
 
+#include <cstdint>
+#include <initializer_list>
+
 #include "../../include/vmbd_solver.h"
 #include "capi_util.h"
 #include "prof.h"
@@ -118,25 +121,51 @@ __global__ __launch_bounds__(TB) void k_reduce_out(const double* partials, int n
   }
 }
 
+__device__ __forceinline__ void pbb_acc(double (&v)[3], float xv, float ox, float gv, float og, int unconstrained) {
+  float dx = xv - ox;
+  float dg = gv - og;
+  if (!unconstrained && xv == 0.f && gv > 0.f) {
+    dx = 0.f;
+    dg = 0.f;
+  }
+  v[0] += (double)dx * dx;
+  v[1] += (double)dx * dg;
+  v[2] += (double)dg * dg;
+}
+
+__device__ __forceinline__ void pbb_acc4(double (&v)[3], float4 a, float4 b, float4 c, float4 d, int unc) {
+  pbb_acc(v, a.x, b.x, c.x, d.x, unc);
+  pbb_acc(v, a.y, b.y, c.y, d.y, unc);
+  pbb_acc(v, a.z, b.z, c.z, d.z, unc);
+  pbb_acc(v, a.w, b.w, c.w, d.w, unc);
+}
+
 // PBB, stage 1 (optGPU.py:202-207 + 222-225): dx = x - oldx, dg = g - oldg,
 // zeroed where x == 0 and g > 0 (clip2bound, optGPU.py:129-132) unless
-// unconstrained; partials of sum dx^2, sum dx dg, sum dg^2.
+// unconstrained; partials of sum dx^2, sum dx dg, sum dg^2.  n4 > 0: the
+// first 4*n4 elements are read as float4 (all operands 16-byte aligned), two
+// chunks per thread in flight; the rest element by element.
 __global__ __launch_bounds__(TB) void k_pbb_stats(const float* x, const float* oldx, const float* g,
-                                                  const float* oldg, long long n, int unconstrained,
+                                                  const float* oldg, long long n, long long n4, int unconstrained,
                                                   double* partials) {
   double v[3] = {0.0, 0.0, 0.0};
-  for (long long e = (long long)blockIdx.x * TB + threadIdx.x; e < n; e += (long long)gridDim.x * TB) {
-    const float xv = x[e], gv = g[e];
-    float dx = xv - oldx[e];
-    float dg = gv - oldg[e];
-    if (!unconstrained && xv == 0.f && gv > 0.f) {
-      dx = 0.f;
-      dg = 0.f;
+  const long long stride = (long long)gridDim.x * TB;
+  const long long tid = (long long)blockIdx.x * TB + threadIdx.x;
+  const float4 *x4 = (const float4*)x, *ox4 = (const float4*)oldx, *g4 = (const float4*)g, *og4 = (const float4*)oldg;
+  for (long long e = tid; e < n4; e += 2 * stride) {
+    const long long e1 = e + stride;
+    const float4 a = x4[e], b = ox4[e], c = g4[e], d = og4[e];
+    float4 a1 = make_float4(0.f, 0.f, 0.f, 0.f), b1 = a1, c1 = a1, d1 = a1;
+    if (e1 < n4) {
+      a1 = x4[e1];
+      b1 = ox4[e1];
+      c1 = g4[e1];
+      d1 = og4[e1];
     }
-    v[0] += (double)dx * dx;
-    v[1] += (double)dx * dg;
-    v[2] += (double)dg * dg;
+    pbb_acc4(v, a, b, c, d, unconstrained);
+    if (e1 < n4) pbb_acc4(v, a1, b1, c1, d1, unconstrained);
   }
+  for (long long e = 4 * n4 + tid; e < n; e += stride) pbb_acc(v, x[e], oldx[e], g[e], oldg[e], unconstrained);
   block_sum<3>(v);
   if (threadIdx.x == 0) {
     partials[blockIdx.x] = v[0];
@@ -149,19 +178,39 @@ __global__ __launch_bounds__(TB) void k_pbb_stats(const float* x, const float* o
 // sum dx^2/(1e-5 + sum dx dg); BB2 on odd, sum dx dg/(1e-5 + sum dg^2);
 // xnew = x - step g, projected onto x >= 0 unless unconstrained.  Every block
 // reduces the stage-1 partials itself (same order -> same step everywhere).
+__device__ __forceinline__ float pbb_new(float xv, float gv, float step, int unconstrained) {
+  const float v = xv - gv * step;
+  return (!unconstrained && v < 0.f) ? 0.f : v;
+}
+
+// n4 as in k_pbb_stats; the operands are loaded before the partials are
+// reduced, so their latency overlaps the reduction.
 __global__ __launch_bounds__(TB) void k_pbb_update(float* xnew, const float* x, const float* g, long long n,
-                                                   const double* partials, int nb, int iter, int unconstrained,
-                                                   float* step_out) {
+                                                   long long n4, const double* partials, int nb, int iter,
+                                                   int unconstrained, float* step_out) {
+  const long long stride = (long long)gridDim.x * TB;
+  const long long tid = (long long)blockIdx.x * TB + threadIdx.x;
+  const float4 *x4 = (const float4*)x, *g4 = (const float4*)g;
+  float4 a = make_float4(0.f, 0.f, 0.f, 0.f), c = a;
+  if (tid < n4) {
+    a = x4[tid];
+    c = g4[tid];
+  }
   double s[3];
   reduce_partials<3>(partials, nb, s);
   const float sxx = (float)s[0], sxg = (float)s[1], sgg = (float)s[2];
   const float step = (iter % 2 == 0) ? sxx / (1e-5f + sxg) : sxg / (1e-5f + sgg);
   if (blockIdx.x == 0 && threadIdx.x == 0 && step_out) *step_out = step;
-  for (long long e = (long long)blockIdx.x * TB + threadIdx.x; e < n; e += (long long)gridDim.x * TB) {
-    float v = x[e] - g[e] * step;
-    if (!unconstrained) v = v < 0.f ? 0.f : v;
-    xnew[e] = v;
+  float4* o4 = (float4*)xnew;
+  for (long long e = tid; e < n4; e += stride) {
+    if (e != tid) {
+      a = x4[e];
+      c = g4[e];
+    }
+    o4[e] = make_float4(pbb_new(a.x, c.x, step, unconstrained), pbb_new(a.y, c.y, step, unconstrained),
+                        pbb_new(a.z, c.z, step, unconstrained), pbb_new(a.w, c.w, step, unconstrained));
   }
+  for (long long e = 4 * n4 + tid; e < n; e += stride) xnew[e] = pbb_new(x[e], g[e], step, unconstrained);
 }
 
 // PBB start (optGPU.py:256-297): the iterate is the start point with the
@@ -318,6 +367,14 @@ __global__ __launch_bounds__(TB) void k_tv_grad(float* grad, const float* x, lon
 using namespace vmbd;
 
 namespace {
+// float4 chunks of an n-element operation whose operands are all 16-byte
+// aligned (0 otherwise: element loop only)
+inline long long vec4_chunks(long long n, std::initializer_list<const void*> ps) {
+  for (const void* p : ps)
+    if (reinterpret_cast<uintptr_t>(p) & 15) return 0;
+  return n / 4;
+}
+
 inline int grid_for(long long n) {
   long long b = (n + TB - 1) / TB;
   if (b > VMBD_MAX_PARTIALS) b = VMBD_MAX_PARTIALS;
@@ -372,7 +429,9 @@ int vmbd_pbb_stats(const float* x, const float* oldx, const float* g, const floa
   hipStream_t st = (hipStream_t)stream;
   const int nb = grid_for(n);
   ProfScope ps("pbb_stats", st);
-  hipLaunchKernelGGL(k_pbb_stats, dim3(nb), dim3(TB), 0, st, x, oldx, g, oldg, (long long)n, unconstrained, partials);
+  const long long n4 = vec4_chunks(n, {x, oldx, g, oldg});
+  hipLaunchKernelGGL(k_pbb_stats, dim3(nb), dim3(TB), 0, st, x, oldx, g, oldg, (long long)n, n4, unconstrained,
+                     partials);
   if (sums) hipLaunchKernelGGL(k_reduce_out, dim3(1), dim3(TB), 0, st, partials, nb, 3, sums);
   SOLVER_DONE("vmbd_pbb_stats");
 }
@@ -384,7 +443,8 @@ int vmbd_pbb_update(float* xnew, const float* x, const float* g, int64_t n, cons
   hipStream_t st = (hipStream_t)stream;
   const int nb = grid_for(n);
   ProfScope ps("pbb_update", st);
-  hipLaunchKernelGGL(k_pbb_update, dim3(nb), dim3(TB), 0, st, xnew, x, g, (long long)n, partials, nb, iter,
+  const long long n4 = vec4_chunks(n, {xnew, x, g});
+  hipLaunchKernelGGL(k_pbb_update, dim3(nb), dim3(TB), 0, st, xnew, x, g, (long long)n, n4, partials, nb, iter,
                      unconstrained, step_out);
   SOLVER_DONE("vmbd_pbb_update");
 }
@@ -397,8 +457,11 @@ int vmbd_pbb_step(float* xnew, const float* x, const float* oldx, const float* g
   hipStream_t st = (hipStream_t)stream;
   const int nb = grid_for(n);
   ProfScope ps("pbb_step", st);
-  hipLaunchKernelGGL(k_pbb_stats, dim3(nb), dim3(TB), 0, st, x, oldx, g, oldg, (long long)n, unconstrained, partials);
-  hipLaunchKernelGGL(k_pbb_update, dim3(nb), dim3(TB), 0, st, xnew, x, g, (long long)n, partials, nb, iter,
+  const long long n4s = vec4_chunks(n, {x, oldx, g, oldg});
+  const long long n4u = vec4_chunks(n, {xnew, x, g});
+  hipLaunchKernelGGL(k_pbb_stats, dim3(nb), dim3(TB), 0, st, x, oldx, g, oldg, (long long)n, n4s, unconstrained,
+                     partials);
+  hipLaunchKernelGGL(k_pbb_update, dim3(nb), dim3(TB), 0, st, xnew, x, g, (long long)n, n4u, partials, nb, iter,
                      unconstrained, step_out);
   SOLVER_DONE("vmbd_pbb_step");
 }
