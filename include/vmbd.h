@@ -8,6 +8,12 @@
  * message of the last failure on the calling thread is in vmbd_last_error().
  * No entry point synchronises the device.
  *
+ * A plan belongs to the device that was current when it was created; calls
+ * made while another device is current fail with VMBD_E_ARG.  A plan's
+ * workspaces serve one call at a time: calls on one plan must be ordered
+ * (one stream, or streams the caller synchronises); separate plans are
+ * independent.
+ *
  * Reference interfaces replaced (files of the reference repository):
  *   plan + windows      imagetools.py:398-454 (win2winaux), olaGPU.py:95-166
  *   operator            olaGPU.py:170-311 (OlaGPU.cnv / OlaGPU.cnvtp) and the

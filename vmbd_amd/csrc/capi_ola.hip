@@ -239,6 +239,11 @@ int check_plan(const vmbd_plan* p, int nframes) {
   if (!p) return vmbd_fail(VMBD_E_ARG, "null plan");
   if (nframes < 1 || nframes > p->max_frames)
     return vmbd_fail(VMBD_E_ARG, "nframes=%d outside [1, max_frames=%d]", nframes, p->max_frames);
+  // the plan's workspaces live on the device it was created on
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev != p->device)
+    return vmbd_fail(VMBD_E_ARG, "plan belongs to device %d, the calling thread's current device is %d", p->device,
+                     dev);
   return VMBD_OK;
 }
 
