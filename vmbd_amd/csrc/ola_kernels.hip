@@ -465,40 +465,38 @@ __host__ __device__ int ola_window_offset_floats(const Geo& g, const OlaDst& d);
 
 // Window rows of output row u: row u - m*hop0 of every window class of every
 // covering patch row m (pair pr = mi*nslot + slot), zero where the row lies
-// outside the window or the slot is unused.  Each wavefront takes pairs
-// pr = w + 4k; a lane loads columns lane + 64j of two pairs before writing
-// any of them.
+// outside the window or the slot is unused.  Wavefront w stages pairs
+// pr = w + 4k by direct global-to-LDS loads (lane l of a load writes LDS
+// float l past the wavefront-uniform base): no registers hold the rows, so
+// the spectrum rows are fetched while they are in flight.  The caller waits
+// for the loads (s_waitcnt) and a workgroup barrier before reading them.
+typedef __attribute__((address_space(3))) void lds_void_t;
+
 template <int TW>
 __device__ __forceinline__ void ola_stage_windows(float* wl, const Geo& g, const OlaDst& d, int Mc, int mlo, int u) {
   constexpr int NW = WG / 64;
   constexpr int JU = (TW + 63) / 64;  // sw1 <= TW
   const int npair = Mc * d.nslot;
   const int lane = threadIdx.x & 63;
-  for (int p0 = (int)(threadIdx.x >> 6); p0 < npair; p0 += 2 * NW) {
-    float w[2][JU];
-#pragma unroll
-    for (int k = 0; k < 2; ++k) {
-      const int pr = p0 + k * NW;
-      const int mi = pr / d.nslot;
-      const int slot = pr - mi * d.nslot;
-      const int m = mlo + mi;
-      const int r = u - m * g.hop0;
-      const int cls = pr < npair ? d.row_class[m * d.nslot + slot] : -1;
-      const bool ok = cls >= 0 && r >= 0 && r < g.sw0;
-      const float* src = d.win + ((size_t)(ok ? cls : 0) * g.sw0 + (ok ? r : 0)) * g.sw1;
+  for (int pr = (int)(threadIdx.x >> 6); pr < npair; pr += NW) {
+    const int mi = pr / d.nslot;
+    const int slot = pr - mi * d.nslot;
+    const int m = mlo + mi;
+    const int r = u - m * g.hop0;
+    const int cls = d.row_class[m * d.nslot + slot];
+    float* dst = wl + pr * g.sw1;
+    if (cls >= 0 && r >= 0 && r < g.sw0) {
+      const float* src = d.win + ((size_t)cls * g.sw0 + r) * g.sw1;
 #pragma unroll
       for (int j = 0; j < JU; ++j) {
         const int c = lane + 64 * j;
-        w[k][j] = (ok && c < g.sw1) ? src[c] : 0.f;
+        if (c < g.sw1) __builtin_amdgcn_global_load_lds((const void*)(src + c), (lds_void_t*)(dst + 64 * j), 4, 0, 0);
       }
-    }
-#pragma unroll
-    for (int k = 0; k < 2; ++k) {
-      const int pr = p0 + k * NW;
+    } else {
 #pragma unroll
       for (int j = 0; j < JU; ++j) {
         const int c = lane + 64 * j;
-        if (pr < npair && c < g.sw1) wl[pr * g.sw1 + c] = w[k][j];
+        if (c < g.sw1) dst[c] = 0.f;
       }
     }
   }
@@ -546,10 +544,7 @@ __global__ __launch_bounds__(WG) void k_rows_c2r_ola(Geo g, OlaDst d, const floa
   // x-adjoint: the window rows this output row needs, one per (covering patch
   // row, window class of that row), staged in LDS past the gather buffer
   float* wl = (float*)(dyn) + ola_window_offset_floats(g, d);
-  if (d.win) {
-    ola_stage_windows<TW>(wl, g, d, Mc, mlo, u);
-    __syncthreads();
-  }
+  if (d.win) ola_stage_windows<TW>(wl, g, d, Mc, mlo, u);  // waited for before the first rowbuf write
 
   const float inv_hop1 = 1.0f / (float)g.hop1;
   // streamed launches: this row's accumulators in LDS past the staged
@@ -585,7 +580,8 @@ __global__ __launch_bounds__(WG) void k_rows_c2r_ola(Geo g, OlaDst d, const floa
 #pragma unroll
       for (int mm = 0; mm < 16; ++mm) v[mm] = make_float2(0.f, 0.f);
     }
-    if (STREAM || d.alias) __syncthreads();  // slices are about to be overwritten by rowbuf
+    if (d.win && base == 0) __builtin_amdgcn_s_waitcnt(0);  // this wavefront's window loads have landed
+    if (STREAM || d.alias || (d.win && base == 0)) __syncthreads();  // slices about to become rowbuf; windows staged
     if (act) {
       float* ba = rowbuf + (STREAM ? 2 * tr : mi * g.csf1 + na) * d.ext1;
       float* bb = rowbuf + (STREAM ? 2 * tr + 1 : mi * g.csf1 + nb) * d.ext1;
