@@ -39,4 +39,7 @@ GEOMS = [
     ((120, 100), (3, 4), (6, 8), 0.5),
     ((200, 180), (4, 3), (11, 13), 0.25),
     ((48, 40), (1, 1), (3, 3), 0.5),
+    # PSF rows beyond three slots per thread (17 > 3 * 64/16): the column
+    # passes' unpruned transforms
+    ((60, 60), (3, 3), (17, 17), 0.5),
 ]

@@ -265,10 +265,19 @@ __device__ __forceinline__ void stockham_pass(float2 (&v)[16], float2* sl, int t
 // Full complex FFT of length N on the slice `sl` (FftLds<N>::floats2 float2).
 // v: input mapping on entry, output mapping on exit.  S=-1 forward, +1 inverse
 // (unscaled).  Must be called by all TPT threads of the transform.
-template <int N, int S>
+// NZ < 16: the caller guarantees x[t + TPT*r] = 0 for r >= NZ; those slots are
+// taken as constant zeros, so the first pass shrinks to the arithmetic the
+// nonzero inputs need (outputs a caller never reads are removed the same way
+// by the compiler once the call is inlined).
+template <int N, int S, int NZ = 16>
 __device__ __forceinline__ void fft(float2 (&v)[16], float2* sl, int t, const Twiddles<N>& T,
                                     const float2* __restrict__ tw) {
   static_assert(N >= 16 && N <= 1024 && (N & (N - 1)) == 0, "N must be a power of two in [16,1024]");
+  static_assert(NZ >= 1 && NZ <= 16, "NZ in [1,16]");
+  if constexpr (NZ < 16) {
+#pragma unroll
+    for (int r = NZ; r < 16; ++r) v[r] = make_float2(0.f, 0.f);
+  }
   // pass 0: radix 16, Ns = 1, j = t, no twiddles; output index t*16 + r
   dft16<S>(v);
   if constexpr (N == 16) {

@@ -253,10 +253,16 @@ hipError_t launch_rows_r2c(const Geo& g, const RowSrc& s, float2* out, const flo
 // ----------------------------------------------------------------------------
 // cols
 // ----------------------------------------------------------------------------
-template <int TH>
+// NZ < 16: only slots r < NZ can hold stored rows (rlo + nr <= NZ*TPT); the
+// rest are zero without a load
+template <int TH, int NZ = 16>
 __device__ __forceinline__ void col_load(float2 (&v)[16], const ColIn& in, int p, int c, int NB, int t, bool active,
                                          int grp, int row0) {
   constexpr int TPT = TH / 16;
+  if constexpr (NZ < 16) {
+#pragma unroll
+    for (int r = NZ; r < 16; ++r) v[r] = make_float2(0.f, 0.f);
+  }
   if (in.spec) {
     const float2* col = in.p + ((size_t)p * NB + c) * TH;
 #pragma unroll
@@ -265,7 +271,7 @@ __device__ __forceinline__ void col_load(float2 (&v)[16], const ColIn& in, int p
     const float2* col = in.shared ? in.p + ((size_t)grp * NB + c) * in.pitch + row0
                                   : in.p + ((size_t)p * NB + c) * in.pitch;
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
+    for (int r = 0; r < NZ; ++r) {
       const int rr = t + TPT * r - in.rlo;
       v[r] = (active && rr >= 0 && rr < in.nr) ? col[rr] : make_float2(0.f, 0.f);
     }
@@ -275,13 +281,15 @@ __device__ __forceinline__ void col_load(float2 (&v)[16], const ColIn& in, int p
 // stage rows [0,nrows) of the strip in LDS and store row-major [P][nrows][NBP].
 // The staging area aliases the FFT slices: barriers on both sides, except
 // after the kernel's last store (LAST), where no slice is used again.
-template <int TH, int CS, bool LAST = false>
+// MM < 16: the caller guarantees nrows <= MM*TPT, so slots m >= MM are never
+// stored (and the transform's outputs there are dead code).
+template <int TH, int CS, bool LAST = false, int MM = 16>
 __device__ __forceinline__ void col_store_rows(const float2 (&v)[16], float2* st, const ColOut& o, int p, int strip,
                                                int NB, int NBP, int cc, int t) {
   constexpr int TPT = TH / 16;
   __syncthreads();  // all transforms of the workgroup are done with the slices
 #pragma unroll
-  for (int m = 0; m < 16; ++m) {
+  for (int m = 0; m < MM; ++m) {
     const int r = t + TPT * m;
     if (r < o.nrows) st[r * (CS + 1) + cc] = v[m];
   }
@@ -328,8 +336,18 @@ __global__ __launch_bounds__(WG, (TH >= 512 ? 4 : 1)) void k_cols(Geo g, ColIn a
   twiddles_init<TH, true>(T, t, tw);
 
   float2 A[16];
-  col_load<TH>(A, a, p, c, NB, t, active, grp, row0);
-  if (!a.spec && wact) fft<TH, -1>(A, sl, t, T, tw);
+  // operands of at most 3*TPT stored rows (PSF rows) take a transform whose
+  // first pass knows the other slots are zero; so do stores of at most
+  // 3*TPT rows (f-adjoint) for the inverse transform's last pass.  Not in
+  // the 512-point two-adjoint pass, whose registers the extra paths spill.
+  constexpr bool PRUNE = !(TH >= 512 && OP == COL_ADJ2);
+  if (PRUNE && (OP == COL_FWD || OP == COL_SPEC) && !a.spec && a.rlo + a.nr <= 3 * TPT) {
+    col_load<TH, 3>(A, a, p, c, NB, t, active, grp, row0);
+    if (wact) fft<TH, -1, 3>(A, sl, t, T, tw);
+  } else {
+    col_load<TH>(A, a, p, c, NB, t, active, grp, row0);
+    if (!a.spec && wact) fft<TH, -1>(A, sl, t, T, tw);
+  }
   if constexpr (OP == COL_SPEC) {
     if (active) {
       float2* col = spec_out + ((size_t)p * NB + c) * TH;
@@ -338,20 +356,33 @@ __global__ __launch_bounds__(WG, (TH >= 512 ? 4 : 1)) void k_cols(Geo g, ColIn a
     }
   } else {
     float2 B[16];
-    col_load<TH>(B, b, p, c, NB, t, active, grp, row0);
-    if (!b.spec && wact) fft<TH, -1>(B, sl, t, T, tw);
+    if (PRUNE && !b.spec && b.rlo + b.nr <= 3 * TPT) {
+      col_load<TH, 3>(B, b, p, c, NB, t, active, grp, row0);
+      if (wact) fft<TH, -1, 3>(B, sl, t, T, tw);
+    } else {
+      col_load<TH>(B, b, p, c, NB, t, active, grp, row0);
+      if (!b.spec && wact) fft<TH, -1>(B, sl, t, T, tw);
+    }
 #pragma unroll
     for (int m = 0; m < 16; ++m)
       B[m] = cscale((OP == COL_FWD) ? cmul(A[m], B[m]) : cmulc(A[m], B[m]), scale);
-    if (wact) fft<TH, +1>(B, sl, t, T, tw);
-    col_store_rows<TH, CS, OP != COL_ADJ2>(B, sm, o1, p, strip, NB, g.NBP, cc, t);
+    if (OP == COL_ADJ1 && o1.nrows <= 3 * TPT) {
+      if (wact) fft<TH, +1>(B, sl, t, T, tw);
+      col_store_rows<TH, CS, true, 3>(B, sm, o1, p, strip, NB, g.NBP, cc, t);
+    } else {
+      if (wact) fft<TH, +1>(B, sl, t, T, tw);
+      col_store_rows<TH, CS, OP != COL_ADJ2>(B, sm, o1, p, strip, NB, g.NBP, cc, t);
+    }
     if constexpr (OP == COL_ADJ2) {
       col_load<TH>(B, c3, p, c, NB, t, active, grp, row0);
       if (!c3.spec && wact) fft<TH, -1>(B, sl, t, T, tw);
 #pragma unroll
       for (int m = 0; m < 16; ++m) B[m] = cscale(cmulc(A[m], B[m]), scale);
       if (wact) fft<TH, +1>(B, sl, t, T, tw);
-      col_store_rows<TH, CS, true>(B, sm, o2, p, strip, NB, g.NBP, cc, t);
+      if (PRUNE && o2.nrows <= 3 * TPT)
+        col_store_rows<TH, CS, true, 3>(B, sm, o2, p, strip, NB, g.NBP, cc, t);
+      else
+        col_store_rows<TH, CS, true>(B, sm, o2, p, strip, NB, g.NBP, cc, t);
     }
   }
 }
