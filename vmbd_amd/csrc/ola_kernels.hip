@@ -542,7 +542,9 @@ constexpr int OLA_KJ = 8;  // output columns per thread of streamed launches (O1
 // round instead of every covering patch row.  Otherwise all rows of the
 // output row are gathered after the last round (single-round launches alias
 // that buffer with the slices).
-template <int TW, bool STREAM>
+// WIN: x-adjoint launches (d.win set), instantiated apart so that the
+// forward's code carries no window path.
+template <int TW, bool STREAM, bool WIN>
 __global__ __launch_bounds__(WG) void k_rows_c2r_ola(Geo g, OlaDst d, const float2* __restrict__ tw) {
   constexpr int TPT = TW / 16;
   constexpr int TPW = WG / TPT;
@@ -575,12 +577,12 @@ __global__ __launch_bounds__(WG) void k_rows_c2r_ola(Geo g, OlaDst d, const floa
   // x-adjoint: the window rows this output row needs, one per (covering patch
   // row, window class of that row), staged in LDS past the gather buffer
   float* wl = (float*)(dyn) + ola_window_offset_floats(g, d);
-  if (d.win) ola_stage_windows<TW>(wl, g, d, Mc, mlo, u);  // waited for before the first rowbuf write
+  if constexpr (WIN) ola_stage_windows<TW>(wl, g, d, Mc, mlo, u);  // waited for before the first rowbuf write
 
   const float inv_hop1 = 1.0f / (float)g.hop1;
   // streamed launches: this row's accumulators in LDS past the staged
   // windows; column j belongs to thread j mod WG, so no barrier guards them
-  float* accl = wl + (d.win ? d.mc * d.nslot * g.sw1 : 0);
+  float* accl = wl + (WIN ? d.mc * d.nslot * g.sw1 : 0);
   if constexpr (STREAM)
     for (int j = threadIdx.x; j < d.O1; j += WG) accl[j] = 0.f;
 
@@ -598,7 +600,7 @@ __global__ __launch_bounds__(WG) void k_rows_c2r_ola(Geo g, OlaDst d, const floa
     const float2* ra = rok ? d.src + ((size_t)ka * d.srow + r) * g.NBP : d.src;
     const float2* rb = (rok && hasb) ? ra + (size_t)d.srow * g.NBP : ra;
     // staged window rows of both patches
-    const bool wrow = d.win && rok && r < g.sw0;
+    const bool wrow = WIN && rok && r < g.sw0;
     const float* wa = wrow ? wl + (mi * d.nslot + d.win_slot[ka - kf]) * g.sw1 : wl;
     const float* wbp = (wrow && hasb) ? wl + (mi * d.nslot + d.win_slot[ka + 1 - kf]) * g.sw1 : wa;
     float2 v[16];
@@ -611,8 +613,8 @@ __global__ __launch_bounds__(WG) void k_rows_c2r_ola(Geo g, OlaDst d, const floa
 #pragma unroll
       for (int mm = 0; mm < 16; ++mm) v[mm] = make_float2(0.f, 0.f);
     }
-    if (d.win && base == 0) __builtin_amdgcn_s_waitcnt(0);  // this wavefront's window loads have landed
-    if (STREAM || d.alias || (d.win && base == 0)) __syncthreads();  // slices about to become rowbuf; windows staged
+    if (WIN && base == 0) __builtin_amdgcn_s_waitcnt(0);  // this wavefront's window loads have landed
+    if (STREAM || d.alias || (WIN && base == 0)) __syncthreads();  // slices about to become rowbuf; windows staged
     if (act) {
       float* ba = rowbuf + (STREAM ? 2 * tr : mi * g.csf1 + na) * d.ext1;
       float* bb = rowbuf + (STREAM ? 2 * tr + 1 : mi * g.csf1 + nb) * d.ext1;
@@ -621,7 +623,7 @@ __global__ __launch_bounds__(WG) void k_rows_c2r_ola(Geo g, OlaDst d, const floa
         const int col = t + TPT * mm;
         if (col < d.ext1) {
           float xa = v[mm].x, xb = v[mm].y;
-          if (d.win) {
+          if constexpr (WIN) {
             const bool okc = wrow && col < g.sw1;
             xa = okc ? xa * wa[col] : 0.f;
             xb = okc ? xb * wbp[col] : 0.f;
@@ -727,12 +729,18 @@ static hipError_t c2r_ola_t(const Geo& g, const OlaDst& d, const float2* tw, hip
   const int frames = g.P / g.Pf;
   const size_t shm = ola_shared_bytes(g, d);
   ProfScope ps(d.win ? "rows_c2r_ola_win" : "rows_c2r_ola", st);
-  if (d.stream)
-    hipLaunchKernelGGL((k_rows_c2r_ola<TW, true>), dim3((unsigned)d.O0, (unsigned)frames), dim3(WG), shm, st, g, d,
-                       tw);
-  else
-    hipLaunchKernelGGL((k_rows_c2r_ola<TW, false>), dim3((unsigned)d.O0, (unsigned)frames), dim3(WG), shm, st, g, d,
-                       tw);
+  const dim3 grid((unsigned)d.O0, (unsigned)frames);
+  if (d.win) {
+    if (d.stream)
+      hipLaunchKernelGGL((k_rows_c2r_ola<TW, true, true>), grid, dim3(WG), shm, st, g, d, tw);
+    else
+      hipLaunchKernelGGL((k_rows_c2r_ola<TW, false, true>), grid, dim3(WG), shm, st, g, d, tw);
+  } else {
+    if (d.stream)
+      hipLaunchKernelGGL((k_rows_c2r_ola<TW, true, false>), grid, dim3(WG), shm, st, g, d, tw);
+    else
+      hipLaunchKernelGGL((k_rows_c2r_ola<TW, false, false>), grid, dim3(WG), shm, st, g, d, tw);
+  }
   return hipGetLastError();
 }
 
