@@ -124,5 +124,13 @@ size_t ola_shared_bytes(const Geo& g, const OlaDst& d);
 int ola_single_round(const Geo& g, const OlaDst& d);
 // multi-round launches whose output rows fit the per-thread accumulators
 int ola_stream_rounds(const Geo& g, const OlaDst& d);
+// Staged x-adjoint window rows: thread t of a transform (TPT = Tw/16 threads)
+// weights columns t + TPT*mm; they are kept at floats t*MW + mm of a row of
+// TPT*MW floats, MW = ceil(sw1/TPT) rounded up to a multiple of 4.
+__host__ __device__ inline int ola_win_mw(const Geo& g) {
+  const int TPT = g.Tw / 16;
+  return (((g.sw1 + TPT - 1) / TPT) + 3) & ~3;
+}
+__host__ __device__ inline int ola_win_rw(const Geo& g) { return (g.Tw / 16) * ola_win_mw(g); }
 
 }  // namespace vmbd

@@ -496,41 +496,58 @@ __host__ __device__ int ola_window_offset_floats(const Geo& g, const OlaDst& d);
 
 // Window rows of output row u: row u - m*hop0 of every window class of every
 // covering patch row m (pair pr = mi*nslot + slot), zero where the row lies
-// outside the window or the slot is unused.  Wavefront w stages pairs
-// pr = w + 4k by direct global-to-LDS loads (lane l of a load writes LDS
-// float l past the wavefront-uniform base): no registers hold the rows, so
-// the spectrum rows are fetched while they are in flight.  The caller waits
-// for the loads (s_waitcnt) and a workgroup barrier before reading them.
+// outside the window or the slot is unused.  A staged row keeps the columns
+// t + TPT*mm that thread t of a transform weights at floats t*MW + mm
+// (ola_win_mw), so the thread reads them with 16-byte LDS loads.  The slot
+// of every covering patch, [mi][n], is staged beside them.  Wavefront w
+// stages pairs pr = w + 4k by direct global-to-LDS loads (lane l of a load
+// writes LDS float l past the wavefront-uniform base, from any address): no
+// registers hold the rows, so the spectrum rows are fetched while they are in
+// flight.  The caller waits for the loads (s_waitcnt) and a workgroup
+// barrier before reading them.
 typedef __attribute__((address_space(3))) void lds_void_t;
 
 template <int TW>
-__device__ __forceinline__ void ola_stage_windows(float* wl, const Geo& g, const OlaDst& d, int Mc, int mlo, int u) {
+__device__ __forceinline__ void ola_stage_windows(float* wl, int* wsl, const Geo& g, const OlaDst& d, int Mc, int mlo,
+                                                  int u) {
+  constexpr int TPT = TW / 16;
   constexpr int NW = WG / 64;
-  constexpr int JU = (TW + 63) / 64;  // sw1 <= TW
+  constexpr int JU = (TW + 63) / 64;  // staged row width <= TW
+  const int MW = ola_win_mw(g);
+  const int RW = TPT * MW;
   const int npair = Mc * d.nslot;
   const int lane = threadIdx.x & 63;
-  for (int pr = (int)(threadIdx.x >> 6); pr < npair; pr += NW) {
+  const int w = (int)(threadIdx.x >> 6);
+  for (int pr = w; pr < npair; pr += NW) {
     const int mi = pr / d.nslot;
     const int slot = pr - mi * d.nslot;
     const int m = mlo + mi;
     const int r = u - m * g.hop0;
     const int cls = d.row_class[m * d.nslot + slot];
-    float* dst = wl + pr * g.sw1;
+    float* dst = wl + pr * RW;
     if (cls >= 0 && r >= 0 && r < g.sw0) {
       const float* src = d.win + ((size_t)cls * g.sw0 + r) * g.sw1;
 #pragma unroll
       for (int j = 0; j < JU; ++j) {
-        const int c = lane + 64 * j;
-        if (c < g.sw1) __builtin_amdgcn_global_load_lds((const void*)(src + c), (lds_void_t*)(dst + 64 * j), 4, 0, 0);
+        const int q = lane + 64 * j;  // staged position: thread tt, slot mm
+        const int tt = q / MW;
+        const int c = tt + TPT * (q - tt * MW);
+        if (q < RW && c < g.sw1)
+          __builtin_amdgcn_global_load_lds((const void*)(src + c), (lds_void_t*)(dst + 64 * j), 4, 0, 0);
       }
     } else {
 #pragma unroll
       for (int j = 0; j < JU; ++j) {
-        const int c = lane + 64 * j;
-        if (c < g.sw1) dst[c] = 0.f;
+        const int q = lane + 64 * j;
+        if (q < RW) dst[q] = 0.f;
       }
     }
   }
+  for (int mi = w; mi < Mc; mi += NW)
+    for (int n0 = 0; n0 < g.csf1; n0 += 64)
+      if (n0 + lane < g.csf1)
+        __builtin_amdgcn_global_load_lds((const void*)(d.win_slot + (mlo + mi) * g.csf1 + n0 + lane),
+                                         (lds_void_t*)(wsl + mi * g.csf1 + n0), 4, 0, 0);
 }
 
 
@@ -545,7 +562,7 @@ constexpr int OLA_KJ = 8;  // output columns per thread of streamed launches (O1
 // WIN: x-adjoint launches (d.win set), instantiated apart so that the
 // forward's code carries no window path.
 template <int TW, bool STREAM, bool WIN>
-__global__ __launch_bounds__(WG) void k_rows_c2r_ola(Geo g, OlaDst d, const float2* __restrict__ tw) {
+__global__ __launch_bounds__(WG, (TW >= 512 ? 3 : 1)) void k_rows_c2r_ola(Geo g, OlaDst d, const float2* __restrict__ tw) {
   constexpr int TPT = TW / 16;
   constexpr int TPW = WG / TPT;
   constexpr int SL = FftLds<TW>::floats2;
@@ -577,12 +594,13 @@ __global__ __launch_bounds__(WG) void k_rows_c2r_ola(Geo g, OlaDst d, const floa
   // x-adjoint: the window rows this output row needs, one per (covering patch
   // row, window class of that row), staged in LDS past the gather buffer
   float* wl = (float*)(dyn) + ola_window_offset_floats(g, d);
-  if constexpr (WIN) ola_stage_windows<TW>(wl, g, d, Mc, mlo, u);  // waited for before the first rowbuf write
+  int* wsl = (int*)(wl + (WIN ? d.mc * d.nslot * ola_win_rw(g) : 0));
+  if constexpr (WIN) ola_stage_windows<TW>(wl, wsl, g, d, Mc, mlo, u);  // waited for before the first rowbuf write
 
   const float inv_hop1 = 1.0f / (float)g.hop1;
   // streamed launches: this row's accumulators in LDS past the staged
   // windows; column j belongs to thread j mod WG, so no barrier guards them
-  float* accl = wl + (WIN ? d.mc * d.nslot * g.sw1 : 0);
+  float* accl = (float*)wsl + (WIN ? d.mc * g.csf1 : 0);
   if constexpr (STREAM)
     for (int j = threadIdx.x; j < d.O1; j += WG) accl[j] = 0.f;
 
@@ -599,10 +617,8 @@ __global__ __launch_bounds__(WG) void k_rows_c2r_ola(Geo g, OlaDst d, const floa
     const bool rok = act && r >= 0 && r < d.srow;
     const float2* ra = rok ? d.src + ((size_t)ka * d.srow + r) * g.NBP : d.src;
     const float2* rb = (rok && hasb) ? ra + (size_t)d.srow * g.NBP : ra;
-    // staged window rows of both patches
+    // the patches' rows lie inside their windows
     const bool wrow = WIN && rok && r < g.sw0;
-    const float* wa = wrow ? wl + (mi * d.nslot + d.win_slot[ka - kf]) * g.sw1 : wl;
-    const float* wbp = (wrow && hasb) ? wl + (mi * d.nslot + d.win_slot[ka + 1 - kf]) * g.sw1 : wa;
     float2 v[16];
     // transforms without an item skip their work and never touch LDS, so a
     // single-round launch holds slices for its items only
@@ -618,15 +634,32 @@ __global__ __launch_bounds__(WG) void k_rows_c2r_ola(Geo g, OlaDst d, const floa
     if (act) {
       float* ba = rowbuf + (STREAM ? 2 * tr : mi * g.csf1 + na) * d.ext1;
       float* bb = rowbuf + (STREAM ? 2 * tr + 1 : mi * g.csf1 + nb) * d.ext1;
+      // staged window values of this thread's columns (x-adjoint: ext1 == sw1)
+      const float* wa = wl;
+      const float* wbp = wl;
+      if constexpr (WIN) {
+        const int MW = ola_win_mw(g);
+        const int RW = TPT * MW;
+        if (wrow) {
+          wa = wl + (mi * d.nslot + wsl[mi * g.csf1 + na]) * RW + t * MW;
+          wbp = hasb ? wl + (mi * d.nslot + wsl[mi * g.csf1 + nb]) * RW + t * MW : wa;
+        }
+      }
+      float4 qa = make_float4(0.f, 0.f, 0.f, 0.f), qb = qa;
 #pragma unroll
       for (int mm = 0; mm < 16; ++mm) {
         const int col = t + TPT * mm;
         if (col < d.ext1) {
           float xa = v[mm].x, xb = v[mm].y;
           if constexpr (WIN) {
-            const bool okc = wrow && col < g.sw1;
-            xa = okc ? xa * wa[col] : 0.f;
-            xb = okc ? xb * wbp[col] : 0.f;
+            if ((mm & 3) == 0 && wrow) {
+              qa = *(const float4*)(wa + mm);
+              qb = *(const float4*)(wbp + mm);
+            }
+            const float fa = (mm & 3) == 0 ? qa.x : (mm & 3) == 1 ? qa.y : (mm & 3) == 2 ? qa.z : qa.w;
+            const float fb = (mm & 3) == 0 ? qb.x : (mm & 3) == 1 ? qb.y : (mm & 3) == 2 ? qb.z : qb.w;
+            xa = wrow ? xa * fa : 0.f;
+            xb = wrow ? xb * fb : 0.f;
           }
           ba[col] = xa;
           if (hasb) bb[col] = xb;
@@ -696,14 +729,16 @@ __global__ __launch_bounds__(WG) void k_rows_c2r_ola(Geo g, OlaDst d, const floa
 }
 
 // float offset of the staged windows: past the FFT slices and the gather
-// buffer (which alias each other in single-round launches)
+// buffer (which alias each other in single-round launches), rounded up to
+// 16 bytes for the windows' 16-byte loads
 __host__ __device__ int ola_window_offset_floats(const Geo& g, const OlaDst& d) {
   const int TPT = g.Tw / 16;
   const int TPW = WG / TPT;
   const int nsl = d.alias ? d.mc * ((g.csf1 + 1) / 2) : TPW;  // FFT slices in use
   const int fl = nsl * (g.Tw + g.Tw / 16) * 2;
   const int rb = d.stream ? 2 * TPW * d.ext1 : d.mc * g.csf1 * d.ext1;
-  return (d.alias || d.stream) ? (fl > rb ? fl : rb) : fl + rb;
+  const int off = (d.alias || d.stream) ? (fl > rb ? fl : rb) : fl + rb;
+  return (off + 3) & ~3;
 }
 
 size_t ola_rowbuf_bytes(const Geo& g, const OlaDst& d) {
@@ -712,7 +747,8 @@ size_t ola_rowbuf_bytes(const Geo& g, const OlaDst& d) {
 }
 
 size_t ola_shared_bytes(const Geo& g, const OlaDst& d) {
-  const size_t wl = d.win ? (size_t)d.mc * d.nslot * g.sw1 * sizeof(float) : 0;
+  // staged window rows + slot table
+  const size_t wl = d.win ? ((size_t)d.mc * d.nslot * ola_win_rw(g) + (size_t)d.mc * g.csf1) * sizeof(float) : 0;
   const size_t acc = d.stream ? (size_t)d.O1 * sizeof(float) : 0;
   return (size_t)ola_window_offset_floats(g, d) * sizeof(float) + wl + acc;
 }
