@@ -124,6 +124,8 @@ size_t ola_shared_bytes(const Geo& g, const OlaDst& d);
 int ola_single_round(const Geo& g, const OlaDst& d);
 // multi-round launches whose output rows fit the per-thread accumulators
 int ola_stream_rounds(const Geo& g, const OlaDst& d);
+// threads per workgroup of an OLA launch of `rows` output rows (all frames)
+int ola_wg(const Geo& g, const OlaDst& d, long long rows);
 // Staged x-adjoint window rows: thread t of a transform (TPT = Tw/16 threads)
 // weights columns t + TPT*mm; they are kept at floats t*MW + mm of a row of
 // TPT*MW floats, MW = ceil(sw1/TPT) rounded up to a multiple of 4.

@@ -507,11 +507,11 @@ __host__ __device__ int ola_window_offset_floats(const Geo& g, const OlaDst& d);
 // barrier before reading them.
 typedef __attribute__((address_space(3))) void lds_void_t;
 
-template <int TW>
+template <int TW, int WGS>
 __device__ __forceinline__ void ola_stage_windows(float* wl, int* wsl, const Geo& g, const OlaDst& d, int Mc, int mlo,
                                                   int u) {
   constexpr int TPT = TW / 16;
-  constexpr int NW = WG / 64;
+  constexpr int NW = WGS / 64;  // wavefronts of the workgroup
   constexpr int JU = (TW + 63) / 64;  // staged row width <= TW
   const int MW = ola_win_mw(g);
   const int RW = TPT * MW;
@@ -561,10 +561,13 @@ constexpr int OLA_KJ = 8;  // output columns per thread of streamed launches (O1
 // that buffer with the slices).
 // WIN: x-adjoint launches (d.win set), instantiated apart so that the
 // forward's code carries no window path.
-template <int TW, bool STREAM, bool WIN>
-__global__ __launch_bounds__(WG, (TW >= 512 ? 3 : 1)) void k_rows_c2r_ola(Geo g, OlaDst d, const float2* __restrict__ tw) {
+// WGS: threads per workgroup (ola_wg): single-round launches of a full
+// grid take only the threads their transforms need.
+template <int TW, bool STREAM, bool WIN, int WGS>
+__global__ __launch_bounds__(WGS, (TW >= 512 ? 3 : 1)) void k_rows_c2r_ola(Geo g, OlaDst d, const float2* __restrict__ tw) {
   constexpr int TPT = TW / 16;
-  constexpr int TPW = WG / TPT;
+  constexpr int wgs = WGS;
+  constexpr int TPW = WGS / TPT;
   constexpr int SL = FftLds<TW>::floats2;
   extern __shared__ float2 dyn[];
   float2* fl = dyn;
@@ -595,14 +598,14 @@ __global__ __launch_bounds__(WG, (TW >= 512 ? 3 : 1)) void k_rows_c2r_ola(Geo g,
   // row, window class of that row), staged in LDS past the gather buffer
   float* wl = (float*)(dyn) + ola_window_offset_floats(g, d);
   int* wsl = (int*)(wl + (WIN ? d.mc * d.nslot * ola_win_rw(g) : 0));
-  if constexpr (WIN) ola_stage_windows<TW>(wl, wsl, g, d, Mc, mlo, u);  // waited for before the first rowbuf write
+  if constexpr (WIN) ola_stage_windows<TW, WGS>(wl, wsl, g, d, Mc, mlo, u);  // waited for before the first rowbuf write
 
   const float inv_hop1 = 1.0f / (float)g.hop1;
   // streamed launches: this row's accumulators in LDS past the staged
-  // windows; column j belongs to thread j mod WG, so no barrier guards them
+  // windows; column j belongs to thread j mod wgs, so no barrier guards them
   float* accl = (float*)wsl + (WIN ? d.mc * g.csf1 : 0);
   if constexpr (STREAM)
-    for (int j = threadIdx.x; j < d.O1; j += WG) accl[j] = 0.f;
+    for (int j = threadIdx.x; j < d.O1; j += wgs) accl[j] = 0.f;
 
   for (int base = 0; base < NT; base += TPW) {
     const int tix = base + tr;
@@ -670,7 +673,7 @@ __global__ __launch_bounds__(WG, (TW >= 512 ? 3 : 1)) void k_rows_c2r_ola(Geo g,
       __syncthreads();
       // this round's items tix in [tlo, thi): patch row mi, pair n/2
       const int tlo = base, thi = min(base + TPW, NT);
-      for (int j = threadIdx.x; j < d.O1; j += WG) {
+      for (int j = threadIdx.x; j < d.O1; j += wgs) {
         const int vcol = j + d.oc1;
         if (vcol >= 0) {
           int qh = (int)((float)vcol * inv_hop1);
@@ -691,7 +694,7 @@ __global__ __launch_bounds__(WG, (TW >= 512 ? 3 : 1)) void k_rows_c2r_ola(Geo g,
     }
   }
   if constexpr (STREAM) {
-    for (int j = threadIdx.x; j < d.O1; j += WG) {
+    for (int j = threadIdx.x; j < d.O1; j += wgs) {
       const size_t o = ((size_t)fr * d.O0 + i) * d.O1 + j;
       float a = accl[j];
       if (d.sub) a -= d.sub[o];
@@ -703,7 +706,7 @@ __global__ __launch_bounds__(WG, (TW >= 512 ? 3 : 1)) void k_rows_c2r_ola(Geo g,
   }
   __syncthreads();
   // fixed-order gather over covering patches
-  for (int j = threadIdx.x; j < d.O1; j += WG) {
+  for (int j = threadIdx.x; j < d.O1; j += wgs) {
     const int vcol = j + d.oc1;
     float acc = 0.f;
     if (vcol >= 0) {
@@ -733,7 +736,7 @@ __global__ __launch_bounds__(WG, (TW >= 512 ? 3 : 1)) void k_rows_c2r_ola(Geo g,
 // 16 bytes for the windows' 16-byte loads
 __host__ __device__ int ola_window_offset_floats(const Geo& g, const OlaDst& d) {
   const int TPT = g.Tw / 16;
-  const int TPW = WG / TPT;
+  const int TPW = WG / TPT;  // launches with fewer threads are single-round: TPW unused
   const int nsl = d.alias ? d.mc * ((g.csf1 + 1) / 2) : TPW;  // FFT slices in use
   const int fl = nsl * (g.Tw + g.Tw / 16) * 2;
   const int rb = d.stream ? 2 * TPW * d.ext1 : d.mc * g.csf1 * d.ext1;
@@ -760,22 +763,45 @@ int ola_single_round(const Geo& g, const OlaDst& d) {
 
 int ola_stream_rounds(const Geo& g, const OlaDst& d) { return !ola_single_round(g, d) && d.O1 <= OLA_KJ * WG; }
 
+// threads per workgroup: a single-round launch whose grid fills the GPU
+// many times over takes 128, 192 or 256 threads, the first that covers its
+// transforms (the x-adjoint at Tw 256: 10 transforms, 192 threads, five
+// workgroups per CU instead of four).  Smaller grids keep WG: there the
+// latency of one output row, not the number of resident rows, sets the time.
+int ola_wg(const Geo& g, const OlaDst& d, long long rows) {
+  if (!d.alias || rows < 8 * 1024) return WG;
+  const int need = (g.Tw / 16) * d.mc * ((g.csf1 + 1) / 2);
+  return need <= 128 ? 128 : need <= 192 ? 192 : WG;
+}
+
+// streamed launches always take WG threads (ola_wg)
+template <int TW, int WGS>
+static void c2r_ola_launch(const Geo& g, const OlaDst& d, const float2* tw, hipStream_t st, dim3 grid, size_t shm) {
+  if constexpr (WGS == WG) {
+    if (d.stream) {
+      if (d.win)
+        hipLaunchKernelGGL((k_rows_c2r_ola<TW, true, true, WG>), grid, dim3(WG), shm, st, g, d, tw);
+      else
+        hipLaunchKernelGGL((k_rows_c2r_ola<TW, true, false, WG>), grid, dim3(WG), shm, st, g, d, tw);
+      return;
+    }
+  }
+  if (d.win)
+    hipLaunchKernelGGL((k_rows_c2r_ola<TW, false, true, WGS>), grid, dim3(WGS), shm, st, g, d, tw);
+  else
+    hipLaunchKernelGGL((k_rows_c2r_ola<TW, false, false, WGS>), grid, dim3(WGS), shm, st, g, d, tw);
+}
+
 template <int TW>
 static hipError_t c2r_ola_t(const Geo& g, const OlaDst& d, const float2* tw, hipStream_t st) {
   const int frames = g.P / g.Pf;
   const size_t shm = ola_shared_bytes(g, d);
   ProfScope ps(d.win ? "rows_c2r_ola_win" : "rows_c2r_ola", st);
   const dim3 grid((unsigned)d.O0, (unsigned)frames);
-  if (d.win) {
-    if (d.stream)
-      hipLaunchKernelGGL((k_rows_c2r_ola<TW, true, true>), grid, dim3(WG), shm, st, g, d, tw);
-    else
-      hipLaunchKernelGGL((k_rows_c2r_ola<TW, false, true>), grid, dim3(WG), shm, st, g, d, tw);
-  } else {
-    if (d.stream)
-      hipLaunchKernelGGL((k_rows_c2r_ola<TW, true, false>), grid, dim3(WG), shm, st, g, d, tw);
-    else
-      hipLaunchKernelGGL((k_rows_c2r_ola<TW, false, false>), grid, dim3(WG), shm, st, g, d, tw);
+  switch (ola_wg(g, d, (long long)d.O0 * frames)) {
+    case 128: c2r_ola_launch<TW, 128>(g, d, tw, st, grid, shm); break;
+    case 192: c2r_ola_launch<TW, 192>(g, d, tw, st, grid, shm); break;
+    default: c2r_ola_launch<TW, WG>(g, d, tw, st, grid, shm); break;
   }
   return hipGetLastError();
 }
