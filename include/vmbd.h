@@ -123,7 +123,10 @@ int vmbd_ola_adjoint_xf(vmbd_plan* plan, int32_t nframes, const float* g, const 
 /* Cached spectra, the OlaGPU constructor (olaGPU.py:142-157):
  * kind VMBD_CACHE_PSF  : z = f stack  -> spec = FFT(pad_stack(f))
  * kind VMBD_CACHE_IMAGE: z = image x  -> spec = FFT(chop_mod_pad(x, ws))
- * spec: complex64 device buffer of vmbd_plan_spectrum_elems() elements. */
+ * spec: complex64 device buffer of vmbd_plan_spectrum_elems() elements, in
+ * the operator's layout [P][Tw/2+1][Th] (row bin, then column frequency):
+ * bin 0 holds the column transform of the two real row bins 0 and Tw/2
+ * packed as one complex column (bin0 + i*bin(Tw/2)); bin Tw/2 is zero. */
 int vmbd_spectra(vmbd_plan* plan, int32_t nframes, int32_t kind, const float* z, void* spec,
                  vmbd_stream_t stream);
 

@@ -182,6 +182,13 @@ __global__ __launch_bounds__(WG) void k_rows_r2c(Geo g, RowJobs jobs, const floa
       za[mm] = make_float2(0.5f * (zc.x + zm.x), 0.5f * (zc.y - zm.y));
       zb[mm] = make_float2(0.5f * (zc.y + zm.y), 0.5f * (zm.x - zc.x));
     }
+    // bins 0 and TW/2 of a real row are real: bin 0 carries both,
+    // (X[0], X[TW/2]), so the column pass has TW/2 columns (thread 0 holds
+    // both bins, slots 0 and 8)
+    if (t == 0) {
+      za[0] = make_float2(za[0].x, za[8].x);
+      zb[0] = make_float2(zb[0].x, zb[8].x);
+    }
   } else {
 #pragma unroll
     for (int mm = 0; mm <= 8; ++mm) za[mm] = zb[mm] = make_float2(0.f, 0.f);
@@ -296,17 +303,90 @@ __device__ __forceinline__ void col_store_rows(const float2 (&v)[16], float2* st
   __syncthreads();
   const int c0 = strip * CS;
   const int ci = threadIdx.x & (CS - 1);
-  if (c0 + ci < NB) {
+  if (c0 + ci < NB - 1) {  // bin NB-1 travels packed in bin 0
     float2* dst = o.p + (size_t)p * o.nrows * NBP + c0 + ci;
     for (int r = threadIdx.x / CS; r < o.nrows; r += WG / CS) dst[(size_t)r * NBP] = st[r * (CS + 1) + ci];
   }
   if (!LAST) __syncthreads();  // staging drained before the slices are reused
 }
 
-// 512-point columns: four wavefronts per SIMD (128 VGPRs, a few spilled)
-// beat three without spills
+// Spectrum product of one column, scaled: B = scale * A * B (forward) or
+// scale * A * conj(B) (adjoints).  The column of bin 0 (packed; the first TPT
+// lanes of wavefront 0 of strip 0) carries two real row bins, 0 and TW/2: its
+// transforms are P = D + i N of two Hermitian spectra D, N, taken apart with
+// P(-k) (D = (P(k) + conj P(-k)) / 2, N = (P(k) - conj P(-k)) / 2i),
+// multiplied bin by bin and put together as D' + i N'.  Its operands go to
+// LDS (its own slice and the next column's, which belongs to the same
+// wavefront and is idle between the transforms); after the plain product of
+// every column, all 64 lanes of that wavefront form the packed product in
+// place, each taking pairs of frequencies (k, TH-k), and its lanes read their
+// slots back (SPREAD; otherwise the column's own lanes form it, 16 slots each).  Written
+// as a macro: the same code behind a function boundary takes 32 more VGPRs
+// (four wavefronts per SIMD become three).
+#define VMBD_COL_PRODUCT(CONJ, A, B)                                                                \
+  do {                                                                                              \
+    if (packed) {                                                                                   \
+      _Pragma("unroll") for (int m = 0; m < 16; ++m) {                                              \
+        sm[lds_pad(t + TPT * m)] = A[m];                                                            \
+        sm[SL + lds_pad(t + TPT * m)] = B[m];                                                       \
+      }                                                                                             \
+    }                                                                                               \
+    _Pragma("unroll") for (int m = 0; m < 16; ++m) B[m] =                                           \
+        cscale((CONJ) ? cmulc(A[m], B[m]) : cmul(A[m], B[m]), scale);                               \
+    if (SPREAD && pwave) {                                                                          \
+      /* frequencies k <= TH/2 with their mirror TH-k: one lane reads both */                      \
+      /* pairs of both operands and writes both products in place         */                       \
+      constexpr int NP_ = TH / 128 + 1;                                                             \
+      const int lane_ = threadIdx.x & 63;                                                           \
+      wave_sync();                                                                                  \
+      _Pragma("unroll 1") for (int j = 0; j < NP_; ++j) {                                           \
+        const int k_ = lane_ + 64 * j;                                                              \
+        if (k_ <= TH / 2) {                                                                         \
+          const int kr_ = (TH - k_) & (TH - 1);                                                     \
+          const float2 a_ = sm[lds_pad(k_)], ar_ = sm[lds_pad(kr_)];                                \
+          const float2 b_ = sm[SL + lds_pad(k_)], br_ = sm[SL + lds_pad(kr_)];                      \
+          const float2 ad_ = make_float2(0.5f * (a_.x + ar_.x), 0.5f * (a_.y - ar_.y));             \
+          const float2 an_ = make_float2(0.5f * (a_.y + ar_.y), 0.5f * (ar_.x - a_.x));             \
+          const float2 bd_ = make_float2(0.5f * (b_.x + br_.x), 0.5f * (b_.y - br_.y));             \
+          const float2 bn_ = make_float2(0.5f * (b_.y + br_.y), 0.5f * (br_.x - b_.x));             \
+          const float2 pd_ = (CONJ) ? cmulc(ad_, bd_) : cmul(ad_, bd_);                            \
+          const float2 pn_ = (CONJ) ? cmulc(an_, bn_) : cmul(an_, bn_);                            \
+          /* at -k: D(-k) = conj D(k), N(-k) = conj N(k) */                                         \
+          const float2 pdr_ = cconj(pd_), pnr_ = cconj(pn_);                                        \
+          sm[lds_pad(k_)] = cscale(make_float2(pd_.x - pn_.y, pd_.y + pn_.x), scale);               \
+          if (kr_ != k_) sm[lds_pad(kr_)] = cscale(make_float2(pdr_.x - pnr_.y, pdr_.y + pnr_.x), scale); \
+        }                                                                                           \
+      }                                                                                             \
+      wave_sync();                                                                                  \
+      if (packed) {                                                                                 \
+        _Pragma("unroll") for (int m = 0; m < 16; ++m) B[m] = sm[lds_pad(t + TPT * m)];             \
+      }                                                                                             \
+      wave_sync(); /* the slices are written next by the inverse transform */                      \
+    }                                                                                               \
+    if (!SPREAD && packed) {                                                                        \
+      wave_sync();                                                                                  \
+      _Pragma("unroll") for (int m = 0; m < 16; ++m) {                                              \
+        const int k_ = t + TPT * m;                                                                 \
+        const int kr_ = (TH - k_) & (TH - 1);                                                       \
+        const float2 a_ = sm[lds_pad(k_)], ar_ = sm[lds_pad(kr_)];                                  \
+        const float2 b_ = sm[SL + lds_pad(k_)], br_ = sm[SL + lds_pad(kr_)];                        \
+        const float2 ad_ = make_float2(0.5f * (a_.x + ar_.x), 0.5f * (a_.y - ar_.y));               \
+        const float2 an_ = make_float2(0.5f * (a_.y + ar_.y), 0.5f * (ar_.x - a_.x));               \
+        const float2 bd_ = make_float2(0.5f * (b_.x + br_.x), 0.5f * (b_.y - br_.y));               \
+        const float2 bn_ = make_float2(0.5f * (b_.y + br_.y), 0.5f * (br_.x - b_.x));               \
+        const float2 pd_ = (CONJ) ? cmulc(ad_, bd_) : cmul(ad_, bd_);                              \
+        const float2 pn_ = (CONJ) ? cmulc(an_, bn_) : cmul(an_, bn_);                              \
+        B[m] = cscale(make_float2(pd_.x - pn_.y, pd_.y + pn_.x), scale);                            \
+      }                                                                                             \
+      wave_sync(); /* the slices are written next by the inverse transform */                      \
+    }                                                                                               \
+  } while (0)
+
+// 256- and 512-point columns: four wavefronts per SIMD (at most 128 VGPRs,
+// a few spilled at 512) beat three without spills; the LDS allows four
+// workgroups per CU
 template <int TH, int OP>
-__global__ __launch_bounds__(WG, (TH >= 512 ? 4 : 1)) void k_cols(Geo g, ColIn a, ColIn b, ColIn c3, ColOut o1, ColOut o2, float scale,
+__global__ __launch_bounds__(WG, (TH >= 256 ? 4 : 1)) void k_cols(Geo g, ColIn a, ColIn b, ColIn c3, ColOut o1, ColOut o2, float scale,
                                              float2* __restrict__ spec_out, const float2* __restrict__ tw) {
   constexpr int TPT = TH / 16;
   constexpr int CS = WG / TPT;  // columns per workgroup
@@ -321,9 +401,17 @@ __global__ __launch_bounds__(WG, (TH >= 512 ? 4 : 1)) void k_cols(Geo g, ColIn a
   const int cc = threadIdx.x / TPT;
   const int t = threadIdx.x - cc * TPT;
   const int c = strip * CS + cc;
-  const bool active = c < NB;
+  // columns: bins [0, NB-1); bin 0 carries the real bins 0 and NB-1 packed
+  const bool active = c < NB - 1;
+  const bool packed = c == 0;
+  // wavefront 0 of strip 0 holds the packed column (lanes 0..TPT-1) and
+  // forms its product (uniform per wavefront)
+  const bool pwave = strip == 0 && threadIdx.x < 64;
+  // the two-adjoint pass forms the packed product on the column's own lanes:
+  // spread over the wavefront it would spill (its registers hold three spectra)
+  constexpr bool SPREAD = OP != COL_ADJ2;
   // a wavefront whose columns all lie past the last bin skips its transforms
-  const bool wact = strip * CS + (int)(threadIdx.x >> 6) * (64 / TPT) < NB;
+  const bool wact = strip * CS + (int)(threadIdx.x >> 6) * (64 / TPT) < NB - 1;
   // group and row offset of this patch in shared row transforms
   const int pfr = p / g.Pf;
   const int pk = p - pfr * g.Pf;
@@ -353,6 +441,11 @@ __global__ __launch_bounds__(WG, (TH >= 512 ? 4 : 1)) void k_cols(Geo g, ColIn a
       float2* col = spec_out + ((size_t)p * NB + c) * TH;
 #pragma unroll
       for (int m = 0; m < 16; ++m) col[t + TPT * m] = A[m];
+      if (packed) {  // bin NB-1 travels in bin 0: its column is written as zeros
+        float2* coln = spec_out + ((size_t)p * NB + NB - 1) * TH;
+#pragma unroll
+        for (int m = 0; m < 16; ++m) coln[t + TPT * m] = make_float2(0.f, 0.f);
+      }
     }
   } else {
     float2 B[16];
@@ -363,9 +456,7 @@ __global__ __launch_bounds__(WG, (TH >= 512 ? 4 : 1)) void k_cols(Geo g, ColIn a
       col_load<TH>(B, b, p, c, NB, t, active, grp, row0);
       if (!b.spec && wact) fft<TH, -1>(B, sl, t, T, tw);
     }
-#pragma unroll
-    for (int m = 0; m < 16; ++m)
-      B[m] = cscale((OP == COL_FWD) ? cmul(A[m], B[m]) : cmulc(A[m], B[m]), scale);
+    VMBD_COL_PRODUCT(OP != COL_FWD, A, B);
     if (OP == COL_ADJ1 && o1.nrows <= 3 * TPT) {
       if (wact) fft<TH, +1>(B, sl, t, T, tw);
       col_store_rows<TH, CS, true, 3>(B, sm, o1, p, strip, NB, g.NBP, cc, t);
@@ -376,8 +467,7 @@ __global__ __launch_bounds__(WG, (TH >= 512 ? 4 : 1)) void k_cols(Geo g, ColIn a
     if constexpr (OP == COL_ADJ2) {
       col_load<TH>(B, c3, p, c, NB, t, active, grp, row0);
       if (!c3.spec && wact) fft<TH, -1>(B, sl, t, T, tw);
-#pragma unroll
-      for (int m = 0; m < 16; ++m) B[m] = cscale(cmulc(A[m], B[m]), scale);
+      VMBD_COL_PRODUCT(true, A, B);
       if (wact) fft<TH, +1>(B, sl, t, T, tw);
       if (PRUNE && o2.nrows <= 3 * TPT)
         col_store_rows<TH, CS, true, 3>(B, sm, o2, p, strip, NB, g.NBP, cc, t);
@@ -387,11 +477,13 @@ __global__ __launch_bounds__(WG, (TH >= 512 ? 4 : 1)) void k_cols(Geo g, ColIn a
   }
 }
 
+#undef VMBD_COL_PRODUCT
+
 template <int TH>
 static hipError_t cols_t(const Geo& g, int op, const ColIn& a, const ColIn& b, const ColIn& c, const ColOut& o1,
                          const ColOut& o2, float scale, float2* spec_out, const float2* tw, hipStream_t st) {
   constexpr int CS = WG / (TH / 16);
-  const int strips = (g.NB + CS - 1) / CS;
+  const int strips = (g.NB - 1 + CS - 1) / CS;  // bins [0, NB-1): NB-1 is packed into bin 0
   const dim3 grid((unsigned)strips, (unsigned)g.P);
   static const char* names[4] = {"cols_spec", "cols_fwd", "cols_adj1", "cols_adj2"};
   ProfScope ps(names[op & 3], st);
@@ -433,24 +525,25 @@ hipError_t launch_cols(const Geo& g, int op, const ColIn& a, const ColIn& b, con
 // Bins [0, TW/2] of each row are read with 16-byte loads of two bins (chunk k
 // holds bins 2k, 2k+1; thread t reads chunks t + TPT*i), all issued before
 // any is used.  Rows are stored with a pitch of at least TW/2 + 2 bins, so the
-// last chunk stays inside its row; its second bin is not used.  The full
+// last chunk stays inside its row; its bins are not used: bin 0 holds the
+// real bins 0 and TW/2 of the row, (X[0], X[TW/2]).  The full
 // spectrum Z[c] = A[c] + i B[c] (conjugate-symmetric halves mirrored to
 // TW - c) is written into the transform's LDS slice and slot r takes
 // Z[t + TPT*r].  A missing row (hasa / hasb false) reads as zero.
 template <int TW>
 struct C2r {
   static constexpr int TPT = TW / 16;
-  static constexpr int NCH = TW / 4 + 1;                // two-bin chunks covering [0, TW/2 + 1]
+  static constexpr int NCH = TW / 4;                    // two-bin chunks covering [0, TW/2)
   static constexpr int NI = (NCH + TPT - 1) / TPT;      // chunks per thread
-  static_assert(NI == 5, "chunk count per thread");
+  static_assert(NI == 4, "chunk count per thread");
 };
 
 // global loads of both rows' chunks (all issued before any is used)
 template <int TW>
-__device__ __forceinline__ void c2r_fetch(float4 (&A)[5], float4 (&B)[5], const float2* ra, const float2* rb,
+__device__ __forceinline__ void c2r_fetch(float4 (&A)[4], float4 (&B)[4], const float2* ra, const float2* rb,
                                           bool hasa, bool hasb, int t) {
 #pragma unroll
-  for (int i = 0; i < 5; ++i) {
+  for (int i = 0; i < 4; ++i) {
     const int k = t + C2r<TW>::TPT * i;
     A[i] = B[i] = make_float4(0.f, 0.f, 0.f, 0.f);
     if (hasa && k < C2r<TW>::NCH) __builtin_memcpy(&A[i], ra + 2 * k, 16);
@@ -460,19 +553,25 @@ __device__ __forceinline__ void c2r_fetch(float4 (&A)[5], float4 (&B)[5], const 
 
 // Hermitian extension in the slice and read-back in the input mapping
 template <int TW>
-__device__ __forceinline__ void c2r_stage(float2 (&v)[16], const float4 (&A)[5], const float4 (&B)[5], float2* sl,
+__device__ __forceinline__ void c2r_stage(float2 (&v)[16], const float4 (&A)[4], const float4 (&B)[4], float2* sl,
                                           int t) {
   constexpr int TPT = C2r<TW>::TPT;
 #pragma unroll
-  for (int i = 0; i < 5; ++i) {
+  for (int i = 0; i < 4; ++i) {
     const int k = t + TPT * i;
 #pragma unroll
     for (int e = 0; e < 2; ++e) {
       const int c = 2 * k + e;
       const float2 a = e ? make_float2(A[i].z, A[i].w) : make_float2(A[i].x, A[i].y);
       const float2 b = e ? make_float2(B[i].z, B[i].w) : make_float2(B[i].x, B[i].y);
-      if (c <= TW / 2) sl[lds_pad(c)] = make_float2(a.x - b.y, a.y + b.x);
-      if (c > 0 && c < TW / 2) sl[lds_pad(TW - c)] = make_float2(a.x + b.y, b.x - a.y);
+      if (c == 0) {
+        // bin 0 holds the real bins 0 and TW/2 of each row: (X[0], X[TW/2])
+        sl[lds_pad(0)] = make_float2(a.x, b.x);
+        sl[lds_pad(TW / 2)] = make_float2(a.y, b.y);
+      } else if (c < TW / 2) {
+        sl[lds_pad(c)] = make_float2(a.x - b.y, a.y + b.x);
+        sl[lds_pad(TW - c)] = make_float2(a.x + b.y, b.x - a.y);
+      }
     }
   }
   wave_sync();
@@ -484,7 +583,7 @@ __device__ __forceinline__ void c2r_stage(float2 (&v)[16], const float4 (&A)[5],
 template <int TW>
 __device__ __forceinline__ void c2r_load(float2 (&v)[16], const float2* ra, const float2* rb, bool hasa, bool hasb,
                                          float2* sl, int t) {
-  float4 A[5], B[5];
+  float4 A[4], B[4];
   c2r_fetch<TW>(A, B, ra, rb, hasa, hasb, t);
   c2r_stage<TW>(v, A, B, sl, t);
 }

@@ -11,7 +11,8 @@ Differences from the reference, all deliberate:
   * device arrays are torch tensors; numpy input to ``cnv`` gives numpy
     output, as in the reference (olaGPU.py:254-257);
   * the cached spectrum (``fft_gpu``) is the half spectrum of the real
-    patches in the kernels' layout (P, Tw/2+1, Th) complex64; the
+    patches in the kernels' layout (P, Tw/2+1, Th) complex64 (the real row
+    bins 0 and Tw/2 packed into bin 0, bin Tw/2 zero); the
     closed-form deconvolutions build the reference's full (P, Th, Tw)
     spectrum on first use (``full_spectrum``);
   * ``cnvtp`` of an 'F' instance overlap-adds with the patch extent sw (the
