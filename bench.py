@@ -64,7 +64,9 @@ def kernel_models(plan, B):
     (row transforms of x and f reused by the adjoint)."""
     P = plan.P
     Th, Tw = plan.tile
-    NB = Tw // 2 + 1
+    # complex row bins carried between the passes: 0..Tw/2-1, bin 0 holding
+    # the real bins 0 and Tw/2
+    NB = Tw // 2
     sw0 = plan.sw[0]
     sf0, sf1 = plan.sf
     ext0 = sw0 + sf0 - 1

@@ -201,7 +201,7 @@ __global__ __launch_bounds__(WG) void k_rows_r2c(Geo g, RowJobs jobs, const floa
 #pragma unroll
   for (int mm = 0; mm <= 8; ++mm) {
     const int c = t + TPT * mm;
-    if (c < NBN) {
+    if (c < NBN - 1) {  // bin TW/2 travels in bin 0
       sm[(2 * tr) * NBN + c] = za[mm];
       sm[(2 * tr + 1) * NBN + c] = zb[mm];
     }
@@ -213,7 +213,7 @@ __global__ __launch_bounds__(WG) void k_rows_r2c(Geo g, RowJobs jobs, const floa
   // (an odd row count writes its last pair into the padding row of the pitch)
   const int rp = tid & (TPW - 1);
   if (2 * rp < nrow)
-    for (int c = tid / TPW; c < NBN; c += WG / TPW) {
+    for (int c = tid / TPW; c < NBN - 1; c += WG / TPW) {
       const float2 a = sm[(2 * rp) * NBN + c], b = sm[(2 * rp + 1) * NBN + c];
       *(float4*)(dst + (size_t)c * nrp + 2 * rp) = make_float4(a.x, a.y, b.x, b.y);
     }
