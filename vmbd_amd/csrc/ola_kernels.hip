@@ -407,9 +407,11 @@ __global__ __launch_bounds__(WG, (TH >= 256 ? 4 : 1)) void k_cols(Geo g, ColIn a
   // wavefront 0 of strip 0 holds the packed column (lanes 0..TPT-1) and
   // forms its product (uniform per wavefront)
   const bool pwave = strip == 0 && threadIdx.x < 64;
-  // the two-adjoint pass forms the packed product on the column's own lanes:
-  // spread over the wavefront it would spill (its registers hold three spectra)
-  constexpr bool SPREAD = OP != COL_ADJ2;
+  // the first product spreads the packed column over the wavefront; the
+  // two-adjoint pass forms its second one on the column's own lanes (spread
+  // twice, its registers spill)
+  constexpr bool SPREAD1 = true;
+  constexpr bool SPREAD2 = false;
   // a wavefront whose columns all lie past the last bin skips its transforms
   const bool wact = strip * CS + (int)(threadIdx.x >> 6) * (64 / TPT) < NB - 1;
   // group and row offset of this patch in shared row transforms
@@ -456,7 +458,10 @@ __global__ __launch_bounds__(WG, (TH >= 256 ? 4 : 1)) void k_cols(Geo g, ColIn a
       col_load<TH>(B, b, p, c, NB, t, active, grp, row0);
       if (!b.spec && wact) fft<TH, -1>(B, sl, t, T, tw);
     }
-    VMBD_COL_PRODUCT(OP != COL_FWD, A, B);
+    {
+      constexpr bool SPREAD = SPREAD1;
+      VMBD_COL_PRODUCT(OP != COL_FWD, A, B);
+    }
     if (OP == COL_ADJ1 && o1.nrows <= 3 * TPT) {
       if (wact) fft<TH, +1>(B, sl, t, T, tw);
       col_store_rows<TH, CS, true, 3>(B, sm, o1, p, strip, NB, g.NBP, cc, t);
@@ -467,7 +472,10 @@ __global__ __launch_bounds__(WG, (TH >= 256 ? 4 : 1)) void k_cols(Geo g, ColIn a
     if constexpr (OP == COL_ADJ2) {
       col_load<TH>(B, c3, p, c, NB, t, active, grp, row0);
       if (!c3.spec && wact) fft<TH, -1>(B, sl, t, T, tw);
-      VMBD_COL_PRODUCT(true, A, B);
+      {
+        constexpr bool SPREAD = SPREAD2;
+        VMBD_COL_PRODUCT(true, A, B);
+      }
       if (wact) fft<TH, +1>(B, sl, t, T, tw);
       if (PRUNE && o2.nrows <= 3 * TPT)
         col_store_rows<TH, CS, true, 3>(B, sm, o2, p, strip, NB, g.NBP, cc, t);
