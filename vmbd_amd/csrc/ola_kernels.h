@@ -7,7 +7,8 @@ namespace vmbd {
 
 // Patch-grid geometry of one operator instance, passed by value to kernels.
 //   image x: H x W;  PSF stack f: P x sf0 x sf1;  patch k = m*csf1 + n at
-//   origin (m*hop0, n*hop1);  FFT tile Th x Tw;  half spectrum NB = Tw/2+1.
+//   origin (m*hop0, n*hop1);  FFT tile Th x Tw;  half spectrum NB = Tw/2+1 slots
+//   (the real bins 0 and Tw/2 share slot 0 between the passes).
 //   ext = sw + sf - 1 (support of one patch's linear convolution);
 //   output y: sy0 x sy1 = crop of the OLA buffer at offset (oc0, oc1).
 struct Geo {

@@ -8,10 +8,12 @@
 //                 loads, redistributed to the FFT mapping through LDS), real
 //                 row FFTs (two rows packed in one complex transform), half
 //                 spectrum written bin-major [P][NB][row_pitch(rows)] so the
-//                 column pass reads contiguous runs.  Rows that are all zero
-//                 are never computed.
-//   cols          per (patch, strip of bins): column FFTs of 1..3 inputs, the
-//                 spectrum product (conj for the transposes), inverse column
+//                 column pass reads contiguous runs; the real bins 0 and Tw/2
+//                 of a row share bin 0, (X[0], X[Tw/2]), so Tw/2 columns
+//                 remain.  Rows that are all zero are never computed.
+//   cols          per (patch, strip of 16 columns): column FFTs of 1..3
+//                 inputs, the spectrum product (conj for the transposes; the
+//                 packed column 0 is taken apart and put back), inverse column
 //                 FFT; only the output rows that are needed are stored,
 //                 row-major [P][rows][NBP] (NBP = NB rounded up to 16 bins,
 //                 so a 16-bin strip of a row is one 128-byte line).
@@ -394,8 +396,8 @@ __global__ __launch_bounds__(WG, (TH >= 256 ? 4 : 1)) void k_cols(Geo g, ColIn a
   __shared__ float2 sm[(OP == COL_SPEC) ? CS * SL : cmax<CS * SL, TH * (CS + 1)>::v];
 
   const int NB = g.NB;
-  // strips of one patch are consecutive workgroups: the unaligned 16-bin
-  // segments they write share cache lines, which then fill while resident
+  // strips of one patch are consecutive workgroups; each writes whole
+  // 128-byte lines of its rows (16 bins, row pitch NBP a multiple of 16)
   const int strip = blockIdx.x;
   const int p = blockIdx.y;
   const int cc = threadIdx.x / TPT;
