@@ -190,6 +190,7 @@ OlaDst dst_forward(const vmbd_plan* p, float* y) {
   const Geo& g = p->g;
   OlaDst d{};
   d.out = y;
+  d.scale = inv_scale(g);
   d.O0 = g.sy0;
   d.O1 = g.sy1;
   d.oc0 = g.oc0;
@@ -210,6 +211,7 @@ OlaDst dst_adjoint_x(const vmbd_plan* p, float* xbar) {
   const Geo& g = p->g;
   OlaDst d{};
   d.out = xbar;
+  d.scale = inv_scale(g);
   d.O0 = g.H;
   d.O1 = g.W;
   d.oc0 = 0;
@@ -447,8 +449,8 @@ int vmbd_ola_forward(vmbd_plan* p, int32_t nframes, const float* x, const float*
   VMBD_LAUNCH(launch_rows_r2c_multi(g, srcs, outs, 2, p->tw, st));
   ColOut o1{p->Y1, g.ext0}, o2{nullptr, 0};
   ColIn none{nullptr, 0, 0, 0};
-  VMBD_LAUNCH(launch_cols(g, COL_FWD, raw_in(p->wsX, srcs[0]), raw_in(p->wsF, srcs[1]), none, o1, o2,
-                          inv_scale(g), nullptr, p->tw, st));
+  VMBD_LAUNCH(launch_cols(g, COL_FWD, raw_in(p->wsX, srcs[0]), raw_in(p->wsF, srcs[1]), none, o1, o2, nullptr,
+                          p->tw, st));
   VMBD_LAUNCH(launch_rows_c2r_ola(g, dst_forward(p, y), p->tw, st));
   p->rows_frames = nframes;
   return VMBD_OK;
@@ -468,8 +470,7 @@ int vmbd_ola_adjoint_x(vmbd_plan* p, int32_t nframes, const float* gres, const f
   VMBD_LAUNCH(launch_rows_r2c(g, sf, p->wsF, p->tw, st));
   ColOut o1{p->Y1, g.sw0}, o2{nullptr, 0};
   ColIn none{};
-  VMBD_LAUNCH(launch_cols(g, COL_ADJ1, shared_in(p->wsG, sr), raw_in(p->wsF, sf), none, o1, o2, inv_scale(g),
-                          nullptr, p->tw, st));
+  VMBD_LAUNCH(launch_cols(g, COL_ADJ1, shared_in(p->wsG, sr), raw_in(p->wsF, sf), none, o1, o2, nullptr, p->tw, st));
   VMBD_LAUNCH(launch_rows_c2r_ola(g, dst_adjoint_x(p, xbar), p->tw, st));
   return VMBD_OK;
 }
@@ -488,9 +489,8 @@ int vmbd_ola_adjoint_f(vmbd_plan* p, int32_t nframes, const float* gres, const f
   VMBD_LAUNCH(launch_rows_r2c(g, sx, p->wsX, p->tw, st));
   ColOut o1{p->Y2, g.sf0}, o2{nullptr, 0};
   ColIn none{};
-  VMBD_LAUNCH(launch_cols(g, COL_ADJ1, shared_in(p->wsG, sr), raw_in(p->wsX, sx), none, o1, o2, inv_scale(g),
-                          nullptr, p->tw, st));
-  VMBD_LAUNCH(launch_rows_c2r_stack(g, p->Y2, g.sf0, fbar, g.sf0, g.sf1, p->tw, st));
+  VMBD_LAUNCH(launch_cols(g, COL_ADJ1, shared_in(p->wsG, sr), raw_in(p->wsX, sx), none, o1, o2, nullptr, p->tw, st));
+  VMBD_LAUNCH(launch_rows_c2r_stack(g, p->Y2, g.sf0, fbar, g.sf0, g.sf1, inv_scale(g), p->tw, st));
   return VMBD_OK;
 }
 
@@ -517,10 +517,10 @@ int vmbd_ola_adjoint_xf(vmbd_plan* p, int32_t nframes, const float* gres, const 
     p->rows_frames = nframes;  // wsX/wsF now hold the rows of this x and f
   }
   ColOut o1{p->Y1, g.sw0}, o2{p->Y2, g.sf0};
-  VMBD_LAUNCH(launch_cols(g, COL_ADJ2, shared_in(p->wsG, sr), raw_in(p->wsF, sf), raw_in(p->wsX, sx), o1, o2,
-                          inv_scale(g), nullptr, p->tw, st));
+  VMBD_LAUNCH(launch_cols(g, COL_ADJ2, shared_in(p->wsG, sr), raw_in(p->wsF, sf), raw_in(p->wsX, sx), o1, o2, nullptr,
+                          p->tw, st));
   VMBD_LAUNCH(launch_rows_c2r_ola(g, dst_adjoint_x(p, xbar), p->tw, st));
-  VMBD_LAUNCH(launch_rows_c2r_stack(g, p->Y2, g.sf0, fbar, g.sf0, g.sf1, p->tw, st));
+  VMBD_LAUNCH(launch_rows_c2r_stack(g, p->Y2, g.sf0, fbar, g.sf0, g.sf1, inv_scale(g), p->tw, st));
   return VMBD_OK;
 }
 
@@ -569,7 +569,7 @@ int vmbd_spectra(vmbd_plan* p, int32_t nframes, int32_t kind, const float* z, vo
   VMBD_LAUNCH(launch_rows_r2c(g, s, ws, p->tw, st));
   ColOut none_o{nullptr, 0};
   ColIn none{nullptr, 0, 0, 0};
-  VMBD_LAUNCH(launch_cols(g, COL_SPEC, raw_in(ws, s), none, none, none_o, none_o, 1.f, (float2*)spec, p->tw, st));
+  VMBD_LAUNCH(launch_cols(g, COL_SPEC, raw_in(ws, s), none, none, none_o, none_o, (float2*)spec, p->tw, st));
   return VMBD_OK;
 }
 
@@ -589,8 +589,7 @@ int vmbd_ola_forward_cached_ex(vmbd_plan* p, int32_t nframes, int32_t kind, cons
   VMBD_LAUNCH(launch_rows_r2c(g, s, ws, p->tw, st));
   ColOut o1{p->Y1, g.ext0}, o2{nullptr, 0};
   ColIn none{};
-  VMBD_LAUNCH(launch_cols(g, COL_FWD, raw_in(ws, s), spec_in(spec, g.Th), none, o1, o2, inv_scale(g), nullptr, p->tw,
-                          st));
+  VMBD_LAUNCH(launch_cols(g, COL_FWD, raw_in(ws, s), spec_in(spec, g.Th), none, o1, o2, nullptr, p->tw, st));
   OlaDst d = dst_forward(p, y);
   d.sub = sub;
   d.mul = mul;
@@ -618,17 +617,15 @@ int vmbd_ola_adjoint_cached_ex(vmbd_plan* p, int32_t nframes, int32_t kind, cons
   ColOut o2{nullptr, 0};
   if (kind == VMBD_CACHE_PSF) {
     ColOut o1{p->Y1, g.sw0};
-    VMBD_LAUNCH(launch_cols(g, COL_ADJ1, shared_in(p->wsG, sr), spec_in(spec, g.Th), none, o1, o2, inv_scale(g),
-                            nullptr, p->tw, st));
+    VMBD_LAUNCH(launch_cols(g, COL_ADJ1, shared_in(p->wsG, sr), spec_in(spec, g.Th), none, o1, o2, nullptr, p->tw, st));
     OlaDst d = dst_adjoint_x(p, out);
     d.axz = z;
     d.beta = beta;
     VMBD_LAUNCH(launch_rows_c2r_ola(g, d, p->tw, st));
   } else {
     ColOut o1{p->Y2, g.sf0};
-    VMBD_LAUNCH(launch_cols(g, COL_ADJ1, shared_in(p->wsG, sr), spec_in(spec, g.Th), none, o1, o2, inv_scale(g),
-                            nullptr, p->tw, st));
-    VMBD_LAUNCH(launch_rows_c2r_stack(g, p->Y2, g.sf0, out, g.sf0, g.sf1, p->tw, st, z, beta));
+    VMBD_LAUNCH(launch_cols(g, COL_ADJ1, shared_in(p->wsG, sr), spec_in(spec, g.Th), none, o1, o2, nullptr, p->tw, st));
+    VMBD_LAUNCH(launch_rows_c2r_stack(g, p->Y2, g.sf0, out, g.sf0, g.sf1, inv_scale(g), p->tw, st, z, beta));
   }
   return VMBD_OK;
 }

@@ -99,7 +99,9 @@ struct OlaDst {
   int alias;   // 1: all transforms of a row fit one round; gather buffer aliases the FFT slices
   int stream;  // 1: several rounds, each round's rows (aliasing the FFT slices) gathered into
                //    per-thread column accumulators; needs O1 <= ola_stream_cols()
-  // fused solver arithmetic on the stored value v (each optional):
+  // 1/(Th*Tw) of the unnormalised transforms, applied to the overlap-add sum
+  float scale;
+  // fused solver arithmetic on the stored value v = scale * sum (each optional):
   // v = ((v - sub[o]) * mul[o]) + beta * axz[o], o = output index
   const float* sub;
   const float* mul;
@@ -115,11 +117,11 @@ hipError_t launch_rows_r2c(const Geo& g, const RowSrc& s, float2* out, const flo
 hipError_t launch_rows_r2c_multi(const Geo& g, const RowSrc* s, float2* const* out, int n, const float2* tw,
                                  hipStream_t st);
 hipError_t launch_cols(const Geo& g, int op, const ColIn& a, const ColIn& b, const ColIn& c, const ColOut& o1,
-                       const ColOut& o2, float scale, float2* spec_out, const float2* tw, hipStream_t st);
+                       const ColOut& o2, float2* spec_out, const float2* tw, hipStream_t st);
 hipError_t launch_rows_c2r_ola(const Geo& g, const OlaDst& d, const float2* tw, hipStream_t st);
-// out = inverse row transforms (+ beta * axz when axz != null)
+// out = scale * inverse row transforms (+ beta * axz when axz != null)
 hipError_t launch_rows_c2r_stack(const Geo& g, const float2* src, int srow, float* out, int nrows, int ncols,
-                                 const float2* tw, hipStream_t st, const float* axz = nullptr, float beta = 0.f);
+                                 float scale, const float2* tw, hipStream_t st, const float* axz = nullptr, float beta = 0.f);
 size_t ola_rowbuf_bytes(const Geo& g, const OlaDst& d);
 size_t ola_shared_bytes(const Geo& g, const OlaDst& d);
 int ola_single_round(const Geo& g, const OlaDst& d);

@@ -312,8 +312,9 @@ __device__ __forceinline__ void col_store_rows(const float2 (&v)[16], float2* st
   if (!LAST) __syncthreads();  // staging drained before the slices are reused
 }
 
-// Spectrum product of one column, scaled: B = scale * A * B (forward) or
-// scale * A * conj(B) (adjoints).  The column of bin 0 (packed; the first TPT
+// Spectrum product of one column: B = A * B (forward) or A * conj(B)
+// (adjoints); the inverse transforms' 1/(Th*Tw) is applied to the summed
+// output by the final stores (a power of two: exact).  The column of bin 0 (packed; the first TPT
 // lanes of wavefront 0 of strip 0) carries two real row bins, 0 and TW/2: its
 // transforms are P = D + i N of two Hermitian spectra D, N, taken apart with
 // P(-k) (D = (P(k) + conj P(-k)) / 2, N = (P(k) - conj P(-k)) / 2i),
@@ -334,7 +335,7 @@ __device__ __forceinline__ void col_store_rows(const float2 (&v)[16], float2* st
       }                                                                                             \
     }                                                                                               \
     _Pragma("unroll") for (int m = 0; m < 16; ++m) B[m] =                                           \
-        cscale((CONJ) ? cmulc(A[m], B[m]) : cmul(A[m], B[m]), scale);                               \
+        (CONJ) ? cmulc(A[m], B[m]) : cmul(A[m], B[m]);                                              \
     if (SPREAD && pwave) {                                                                          \
       /* frequencies k <= TH/2 with their mirror TH-k: one lane reads both */                      \
       /* pairs of both operands and writes both products in place         */                       \
@@ -355,8 +356,8 @@ __device__ __forceinline__ void col_store_rows(const float2 (&v)[16], float2* st
           const float2 pn_ = (CONJ) ? cmulc(an_, bn_) : cmul(an_, bn_);                            \
           /* at -k: D(-k) = conj D(k), N(-k) = conj N(k) */                                         \
           const float2 pdr_ = cconj(pd_), pnr_ = cconj(pn_);                                        \
-          sm[lds_pad(k_)] = cscale(make_float2(pd_.x - pn_.y, pd_.y + pn_.x), scale);               \
-          if (kr_ != k_) sm[lds_pad(kr_)] = cscale(make_float2(pdr_.x - pnr_.y, pdr_.y + pnr_.x), scale); \
+          sm[lds_pad(k_)] = make_float2(pd_.x - pn_.y, pd_.y + pn_.x);                              \
+          if (kr_ != k_) sm[lds_pad(kr_)] = make_float2(pdr_.x - pnr_.y, pdr_.y + pnr_.x);          \
         }                                                                                           \
       }                                                                                             \
       wave_sync();                                                                                  \
@@ -378,7 +379,7 @@ __device__ __forceinline__ void col_store_rows(const float2 (&v)[16], float2* st
         const float2 bn_ = make_float2(0.5f * (b_.y + br_.y), 0.5f * (br_.x - b_.x));               \
         const float2 pd_ = (CONJ) ? cmulc(ad_, bd_) : cmul(ad_, bd_);                              \
         const float2 pn_ = (CONJ) ? cmulc(an_, bn_) : cmul(an_, bn_);                              \
-        B[m] = cscale(make_float2(pd_.x - pn_.y, pd_.y + pn_.x), scale);                            \
+        B[m] = make_float2(pd_.x - pn_.y, pd_.y + pn_.x);                                           \
       }                                                                                             \
       wave_sync(); /* the slices are written next by the inverse transform */                      \
     }                                                                                               \
@@ -388,7 +389,7 @@ __device__ __forceinline__ void col_store_rows(const float2 (&v)[16], float2* st
 // a few spilled at 512) beat three without spills; the LDS allows four
 // workgroups per CU
 template <int TH, int OP>
-__global__ __launch_bounds__(WG, (TH >= 256 ? 4 : 1)) void k_cols(Geo g, ColIn a, ColIn b, ColIn c3, ColOut o1, ColOut o2, float scale,
+__global__ __launch_bounds__(WG, (TH >= 256 ? 4 : 1)) void k_cols(Geo g, ColIn a, ColIn b, ColIn c3, ColOut o1, ColOut o2,
                                              float2* __restrict__ spec_out, const float2* __restrict__ tw) {
   constexpr int TPT = TH / 16;
   constexpr int CS = WG / TPT;  // columns per workgroup
@@ -491,7 +492,7 @@ __global__ __launch_bounds__(WG, (TH >= 256 ? 4 : 1)) void k_cols(Geo g, ColIn a
 
 template <int TH>
 static hipError_t cols_t(const Geo& g, int op, const ColIn& a, const ColIn& b, const ColIn& c, const ColOut& o1,
-                         const ColOut& o2, float scale, float2* spec_out, const float2* tw, hipStream_t st) {
+                         const ColOut& o2, float2* spec_out, const float2* tw, hipStream_t st) {
   constexpr int CS = WG / (TH / 16);
   const int strips = (g.NB - 1 + CS - 1) / CS;  // bins [0, NB-1): NB-1 is packed into bin 0
   const dim3 grid((unsigned)strips, (unsigned)g.P);
@@ -499,16 +500,16 @@ static hipError_t cols_t(const Geo& g, int op, const ColIn& a, const ColIn& b, c
   ProfScope ps(names[op & 3], st);
   switch (op) {
     case COL_SPEC:
-      hipLaunchKernelGGL((k_cols<TH, COL_SPEC>), grid, dim3(WG), 0, st, g, a, b, c, o1, o2, scale, spec_out, tw);
+      hipLaunchKernelGGL((k_cols<TH, COL_SPEC>), grid, dim3(WG), 0, st, g, a, b, c, o1, o2, spec_out, tw);
       break;
     case COL_FWD:
-      hipLaunchKernelGGL((k_cols<TH, COL_FWD>), grid, dim3(WG), 0, st, g, a, b, c, o1, o2, scale, spec_out, tw);
+      hipLaunchKernelGGL((k_cols<TH, COL_FWD>), grid, dim3(WG), 0, st, g, a, b, c, o1, o2, spec_out, tw);
       break;
     case COL_ADJ1:
-      hipLaunchKernelGGL((k_cols<TH, COL_ADJ1>), grid, dim3(WG), 0, st, g, a, b, c, o1, o2, scale, spec_out, tw);
+      hipLaunchKernelGGL((k_cols<TH, COL_ADJ1>), grid, dim3(WG), 0, st, g, a, b, c, o1, o2, spec_out, tw);
       break;
     case COL_ADJ2:
-      hipLaunchKernelGGL((k_cols<TH, COL_ADJ2>), grid, dim3(WG), 0, st, g, a, b, c, o1, o2, scale, spec_out, tw);
+      hipLaunchKernelGGL((k_cols<TH, COL_ADJ2>), grid, dim3(WG), 0, st, g, a, b, c, o1, o2, spec_out, tw);
       break;
     default:
       return hipErrorInvalidValue;
@@ -517,14 +518,14 @@ static hipError_t cols_t(const Geo& g, int op, const ColIn& a, const ColIn& b, c
 }
 
 hipError_t launch_cols(const Geo& g, int op, const ColIn& a, const ColIn& b, const ColIn& c, const ColOut& o1,
-                       const ColOut& o2, float scale, float2* spec_out, const float2* tw, hipStream_t st) {
+                       const ColOut& o2, float2* spec_out, const float2* tw, hipStream_t st) {
   switch (g.Th) {
-    case 16: return cols_t<16>(g, op, a, b, c, o1, o2, scale, spec_out, tw, st);
-    case 32: return cols_t<32>(g, op, a, b, c, o1, o2, scale, spec_out, tw, st);
-    case 64: return cols_t<64>(g, op, a, b, c, o1, o2, scale, spec_out, tw, st);
-    case 128: return cols_t<128>(g, op, a, b, c, o1, o2, scale, spec_out, tw, st);
-    case 256: return cols_t<256>(g, op, a, b, c, o1, o2, scale, spec_out, tw, st);
-    case 512: return cols_t<512>(g, op, a, b, c, o1, o2, scale, spec_out, tw, st);
+    case 16: return cols_t<16>(g, op, a, b, c, o1, o2, spec_out, tw, st);
+    case 32: return cols_t<32>(g, op, a, b, c, o1, o2, spec_out, tw, st);
+    case 64: return cols_t<64>(g, op, a, b, c, o1, o2, spec_out, tw, st);
+    case 128: return cols_t<128>(g, op, a, b, c, o1, o2, spec_out, tw, st);
+    case 256: return cols_t<256>(g, op, a, b, c, o1, o2, spec_out, tw, st);
+    case 512: return cols_t<512>(g, op, a, b, c, o1, o2, spec_out, tw, st);
     default: return hipErrorInvalidValue;
   }
 }
@@ -805,7 +806,7 @@ __global__ __launch_bounds__(WGS, (TW >= 512 ? 3 : 1)) void k_rows_c2r_ola(Geo g
   if constexpr (STREAM) {
     for (int j = threadIdx.x; j < d.O1; j += wgs) {
       const size_t o = ((size_t)fr * d.O0 + i) * d.O1 + j;
-      float a = accl[j];
+      float a = accl[j] * d.scale;
       if (d.sub) a -= d.sub[o];
       if (d.mul) a *= d.mul[o];
       if (d.axz) a += d.beta * d.axz[o];
@@ -833,6 +834,7 @@ __global__ __launch_bounds__(WGS, (TW >= 512 ? 3 : 1)) void k_rows_c2r_ola(Geo g
       }
     }
     const size_t o = ((size_t)fr * d.O0 + i) * d.O1 + j;
+    acc *= d.scale;
     if (d.sub) acc -= d.sub[o];
     if (d.mul) acc *= d.mul[o];
     if (d.axz) acc += d.beta * d.axz[o];
@@ -934,7 +936,7 @@ hipError_t launch_rows_c2r_ola(const Geo& g, const OlaDst& d, const float2* tw, 
 // ----------------------------------------------------------------------------
 template <int TW>
 __global__ __launch_bounds__(WG) void k_rows_c2r_stack(Geo g, const float2* __restrict__ src, int srow,
-                                                       float* __restrict__ out, int nrows, int ncols,
+                                                       float* __restrict__ out, int nrows, int ncols, float scale,
                                                        const float2* __restrict__ tw, const float* __restrict__ axz,
                                                        float beta) {
   constexpr int TPT = TW / 16;
@@ -965,10 +967,11 @@ __global__ __launch_bounds__(WG) void k_rows_c2r_stack(Geo g, const float2* __re
       const int col = t + TPT * mm;
       if (col < ncols) {
         const size_t o0 = ((size_t)p * nrows + r0) * ncols + col;
-        out[o0] = axz ? v[mm].x + beta * axz[o0] : v[mm].x;
+        const float v0 = v[mm].x * scale, v1 = v[mm].y * scale;
+        out[o0] = axz ? v0 + beta * axz[o0] : v0;
         if (r1 < nrows) {
           const size_t o1 = o0 + ncols;
-          out[o1] = axz ? v[mm].y + beta * axz[o1] : v[mm].y;
+          out[o1] = axz ? v1 + beta * axz[o1] : v1;
         }
       }
     }
@@ -977,26 +980,26 @@ __global__ __launch_bounds__(WG) void k_rows_c2r_stack(Geo g, const float2* __re
 
 template <int TW>
 static hipError_t c2r_stack_t(const Geo& g, const float2* src, int srow, float* out, int nrows, int ncols,
-                              const float2* tw, hipStream_t st, const float* axz, float beta) {
+                              float scale, const float2* tw, hipStream_t st, const float* axz, float beta) {
   constexpr int TPW = WG / (TW / 16);
   const int blocks = ((nrows + 1) / 2 + TPW - 1) / TPW;
   if (blocks <= 0 || g.P <= 0) return hipSuccess;
   ProfScope ps("rows_c2r_stack", st);
   hipLaunchKernelGGL(k_rows_c2r_stack<TW>, dim3((unsigned)g.P, (unsigned)blocks), dim3(WG), 0, st, g, src, srow,
-                     out, nrows, ncols, tw, axz, beta);
+                     out, nrows, ncols, scale, tw, axz, beta);
   return hipGetLastError();
 }
 
 hipError_t launch_rows_c2r_stack(const Geo& g, const float2* src, int srow, float* out, int nrows, int ncols,
-                                 const float2* tw, hipStream_t st, const float* axz, float beta) {
+                                 float scale, const float2* tw, hipStream_t st, const float* axz, float beta) {
   if (g.NBP < g.Tw / 2 + 2) return hipErrorInvalidValue;  // two-bin loads need a padding bin per row
   switch (g.Tw) {
-    case 16: return c2r_stack_t<16>(g, src, srow, out, nrows, ncols, tw, st, axz, beta);
-    case 32: return c2r_stack_t<32>(g, src, srow, out, nrows, ncols, tw, st, axz, beta);
-    case 64: return c2r_stack_t<64>(g, src, srow, out, nrows, ncols, tw, st, axz, beta);
-    case 128: return c2r_stack_t<128>(g, src, srow, out, nrows, ncols, tw, st, axz, beta);
-    case 256: return c2r_stack_t<256>(g, src, srow, out, nrows, ncols, tw, st, axz, beta);
-    case 512: return c2r_stack_t<512>(g, src, srow, out, nrows, ncols, tw, st, axz, beta);
+    case 16: return c2r_stack_t<16>(g, src, srow, out, nrows, ncols, scale, tw, st, axz, beta);
+    case 32: return c2r_stack_t<32>(g, src, srow, out, nrows, ncols, scale, tw, st, axz, beta);
+    case 64: return c2r_stack_t<64>(g, src, srow, out, nrows, ncols, scale, tw, st, axz, beta);
+    case 128: return c2r_stack_t<128>(g, src, srow, out, nrows, ncols, scale, tw, st, axz, beta);
+    case 256: return c2r_stack_t<256>(g, src, srow, out, nrows, ncols, scale, tw, st, axz, beta);
+    case 512: return c2r_stack_t<512>(g, src, srow, out, nrows, ncols, scale, tw, st, axz, beta);
     default: return hipErrorInvalidValue;
   }
 }
