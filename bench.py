@@ -203,7 +203,7 @@ def mixed_rate(nframes, timed, steps):
         replay()
     ms = timed(replay, steps)
     return {'frames': nframes, 'geometries': [[list(a), list(b), list(c)] for a, b, c in C4_GEOMS],
-            'groups': len(mb.groups), 'ms_per_step': round(ms, 3), 'frames_per_s': round(nframes / (ms * 1e-3), 1),
+            'groups': len(mb.groups), 'streams': mb.streams, 'ms_per_step': round(ms, 3), 'frames_per_s': round(nframes / (ms * 1e-3), 1),
             'mpix_s': round(mb.pixels / (ms * 1e-3) / 1e6, 1), 'graph': True}
 
 

@@ -51,12 +51,15 @@ def test_mixed_batch_vs_oracle_and_single_plans(cuda):
         assert torch.equal(xb1, xb) and torch.equal(fb1, fb), i
 
 
-def test_mixed_batch_graph_replay(cuda):
+@pytest.mark.parametrize('streams', [1, 2, 3])
+def test_mixed_batch_graph_replay(cuda, streams):
+    """The captured step equals the eager one, also with the groups forked
+    over several streams inside the graph."""
     import torch
     from vmbd_amd.batch import MixedBatch
-    print('case mixed batch graph', flush=True)
+    print(f'case mixed batch graph, {streams} streams', flush=True)
     xs, fs, gs = _frames(cuda)
-    mb = MixedBatch(SPECS)
+    mb = MixedBatch(SPECS, streams=streams)
     mb.load(xs, fs, gs)
     mb.step()
     torch.cuda.synchronize()
