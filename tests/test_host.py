@@ -84,3 +84,27 @@ def test_plan_create_rejects_bad_geometry_without_gpu_work():
     g = L.VmbdGeom(64, 64, 5, 5, 2, 2, 48, 48, 24, 24, 64, 64, 0)
     p = h.vmbd_plan_create(ctypes.byref(g), tab.ctypes.data_as(ctypes.c_void_p), 1, idx.ctypes.data_as(ctypes.c_void_p))
     assert not p and b'win_idx' in h.vmbd_last_error()
+
+
+def test_mixed_batch_stream_lanes():
+    """Groups go to streams largest first, each to the stream with the fewest
+    pixels so far; every group lands on exactly one stream."""
+    import torch
+    from vmbd_amd.batch import MixedBatch
+
+    def fake(streams, groups):
+        mb = MixedBatch.__new__(MixedBatch)
+        mb.streams = streams
+        mb.groups = [dict(idx=list(range(n)), x=torch.zeros((n,) + sx)) for n, sx in groups]
+        return mb
+
+    # pixels per group: 4*16, 2*64, 1*16, 3*16, 1*64
+    groups = [(4, (4, 4)), (2, (8, 8)), (1, (4, 4)), (3, (4, 4)), (1, (8, 8))]
+    assert fake(1, groups)._lanes() == [[1, 0, 4, 3, 2]]
+    lanes = fake(2, groups)._lanes()
+    assert lanes == [[1, 3], [0, 4, 2]]
+    assert sorted(g for lane in lanes for g in lane) == list(range(5))
+    # more streams than groups: one group per stream
+    assert fake(8, groups[:3])._lanes() == [[1], [0], [2]]
+    with pytest.raises(ValueError):
+        MixedBatch([], streams=0)

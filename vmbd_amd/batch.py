@@ -27,11 +27,11 @@ from .ola import OlaPlan
 
 class MixedBatch:
     def __init__(self, specs, overlap=0.5, tune=False, exact=True, device=None, streams=3):
-        self.device = torch.device(device) if device is not None else torch.device('cuda', torch.cuda.current_device())
-        self.exact = bool(exact)
         if int(streams) < 1:
             raise ValueError(f'streams must be >= 1, got {streams}')
         self.streams = int(streams)
+        self.device = torch.device(device) if device is not None else torch.device('cuda', torch.cuda.current_device())
+        self.exact = bool(exact)
         self.specs = [(tuple(int(v) for v in sx), (int(c[0]), int(c[1])), (int(s[0]), int(s[1])))
                       for sx, c, s in specs]
         groups = OrderedDict()
