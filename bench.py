@@ -42,7 +42,8 @@ def parse():
     ap.add_argument('--warmup', type=int, default=5)
     ap.add_argument('--frames', type=int, default=32, help='frames per step, batched into the same launches')
     ap.add_argument('--no-graph', action='store_true', help='launch eagerly instead of replaying a HIP graph')
-    ap.add_argument('--cpu-seconds', type=float, default=12.0, help='bounded CPU-baseline sample')
+    ap.add_argument('--cpu-seconds', type=float, default=18.0,
+                    help='bounded CPU-baseline sample (split over its three variants)')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-plain', action='store_true', help='skip the plain per-operator comparator')
     ap.add_argument('--no-mfbd', action='store_true', help='skip the MFBD frames/s measurement')
@@ -99,6 +100,47 @@ def operator_bytes():
     return 4.0 * 3 * (Sx + Sy + Pf)
 
 
+def operator_flops():
+    """Algorithmic flops of fwd + x-adj + f-adj of one frame: P (5 real
+    T^2 FFTs + 9 T^2), real FFT = half of 5 N log2 N for N = T^2 points."""
+    P = CSF[0] * CSF[1]
+    T = 256
+    rfft = 2.5 * T * T * math.log2(T * T)
+    return P * (5 * rfft + 9 * T * T)
+
+
+def attainable_hbm_frac():
+    """HBM fraction the step can reach when it runs at the non-packed FP32
+    VALU peak: t_HBM / t_VALU of the algorithmic bytes and flops."""
+    return (operator_bytes() / (HBM_PEAK_GBS * 1e9)) / (operator_flops() / (VALU_FP32_PEAK * 1e12))
+
+
+def operator_roofline(kernels, B, ms):
+    """Roofline of the dominant kernel on the operator's algorithmic bytes and
+    flops (one fwd + x-adj + f-adj per frame, inputs and outputs once,
+    windows generated in the kernels): the B frames' bytes and flops over
+    the kernel's average launch time, and over the whole step's time.  The
+    step is VALU-issue bound (52 flop per byte against a ridge of 10 for
+    non-packed FP32), so the HBM fraction it can reach is capped at
+    attainable_hbm_frac()."""
+    dom = max(kernels, key=lambda k: kernels[k]['total_us_per_step'])
+    d = kernels[dom]
+    by, fl = operator_bytes() * B, operator_flops() * B
+    t_k = d['avg_us'] * 1e-6
+    gbs = by / t_k / 1e9
+    tf = fl / t_k / 1e12
+    step_gbs = by / (ms * 1e-3) / 1e9
+    step_tf = fl / (ms * 1e-3) / 1e12
+    return dict(kernel=dom, bound='valu', achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit='GB/s',
+                frac=round(gbs / HBM_PEAK_GBS, 4), traffic=measured_traffic(dom, B), avg_us=d['avg_us'],
+                alg_bytes_per_launch=by, alg_flops_per_launch=fl,
+                valu=dict(achieved_tflops=round(tf, 2), peak_nonpacked=VALU_FP32_PEAK,
+                          frac=round(tf / VALU_FP32_PEAK, 4)),
+                step=dict(hbm_gbs=round(step_gbs, 1), hbm_frac=round(step_gbs / HBM_PEAK_GBS, 5),
+                          valu_tflops=round(step_tf, 2), valu_frac=round(step_tf / VALU_FP32_PEAK, 4)),
+                attainable_hbm_frac=round(attainable_hbm_frac(), 4))
+
+
 TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'profiles',
                             'ola_1024_10x10_sf41_b32_hbm_traffic.md')
 
@@ -119,6 +161,28 @@ def measured_traffic(kernel, frames):
         if len(parts) == 3 and parts[0] == 'traffic_bytes_per_launch' and parts[1] == kernel:
             return int(parts[2])
     return None
+
+
+def host_threads():
+    """Threads the CPU baseline may use: OMP_NUM_THREADS when set (the host
+    share of the GPU's node), else the CPUs this process may run on."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        avail = os.cpu_count() or 1
+    env = os.environ.get('OMP_NUM_THREADS', '')
+    return max(1, min(avail, int(env))) if env.isdigit() and int(env) > 0 else avail
+
+
+def cpu_model():
+    try:
+        with open('/proc/cpuinfo') as fh:
+            for ln in fh:
+                if ln.startswith('model name'):
+                    return ln.split(':', 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
 
 
 def plain_form(wa, x, f, g, timed, a):
@@ -304,20 +368,14 @@ def main():
         e = dict(launches_per_step=cnt / a.steps, avg_us=round(avg, 3), total_us_per_step=round(tot / a.steps, 3))
         mm = models.get(name)
         if mm:
-            e['alg_bytes'] = mm['bytes']
-            e['hbm_gbs'] = round(mm['bytes'] / (avg * 1e-6) / 1e9, 1)
-            e['alg_flops'] = mm['flops']
+            # the kernel's own intermediate operands (row/column spectra)
+            e['kernel_bytes'] = mm['bytes']
+            e['kernel_bytes_gbs'] = round(mm['bytes'] / (avg * 1e-6) / 1e9, 1)
+            e['nominal_flops'] = mm['flops']
             e['tflops'] = round(mm['flops'] / (avg * 1e-6) / 1e12, 3)
         kernels[name] = e
-    dom = max(kernels, key=lambda k: kernels[k]['total_us_per_step'])
-    d = kernels[dom]
-    roofline = dict(kernel=dom, bound='hbm', achieved=d.get('hbm_gbs', 0.0), peak=HBM_PEAK_GBS, unit='GB/s',
-                    frac=round(d.get('hbm_gbs', 0.0) / HBM_PEAK_GBS, 4),
-                    traffic=measured_traffic(dom, B), avg_us=d['avg_us'],
-                    valu=dict(achieved_tflops=d.get('tflops', 0.0), peak_nonpacked=VALU_FP32_PEAK,
-                              frac=round(d.get('tflops', 0.0) / VALU_FP32_PEAK, 4)))
+    roofline = operator_roofline(kernels, B, ms)
     op_bytes = operator_bytes() * B
-    step_hbm = op_bytes / (ms * 1e-3) / 1e9
 
     plain = None
     if not a.no_plain:
@@ -344,20 +402,39 @@ def main():
         x64 = np.float32(x_np[0]).astype(np.float64)
         f64 = np.float32(f_np[0]).astype(np.float64)
         g64 = np.float32(g_np[0]).astype(np.float64)
-        n = 0
-        t0 = time.perf_counter()
-        while True:
-            y_ref = O.forward(x64, f64, owa)
-            xb_ref = O.adjoint_x(g64, f64, owa, (H, W), exact=True)
-            fb_ref = O.adjoint_f(g64, x64, owa, SF, exact=True)
-            n += 1
-            if time.perf_counter() - t0 >= a.cpu_seconds:
-                break
-        dt = (time.perf_counter() - t0) / n
-        cpu = dict(value=round(H * W / dt / 1e6, 4), unit='Mpix/s', cores=1, kind='port',
-                   sample=f'{n} x (fwd + x-adj + f-adj) of one 1024^2 frame, FP64 NumPy oracle, single thread, '
-                          f'{dt:.2f} s each on {platform.processor() or platform.machine()} '
-                          f'({os.cpu_count()} host cores visible)')
+        threads = host_threads()
+
+        def cpu_rate(workers, single, seconds):
+            """Mpix/s of the oracle's fwd + x-adj + f-adj on one frame, timed
+            for about `seconds` (at least one step); returns (rate, steps,
+            s/step, outputs of the last step)."""
+            old = O.set_fft(workers, single)
+            try:
+                n, t0 = 0, time.perf_counter()
+                while True:
+                    outs = (O.forward(x64, f64, owa), O.adjoint_x(g64, f64, owa, (H, W), exact=True),
+                            O.adjoint_f(g64, x64, owa, SF, exact=True))
+                    n += 1
+                    if time.perf_counter() - t0 >= seconds:
+                        break
+                dt = (time.perf_counter() - t0) / n
+            finally:
+                O.set_fft(*old)
+            return H * W / dt / 1e6, n, dt, outs
+
+        where = f'{cpu_model()}, {os.cpu_count()} logical cores visible'
+        r1, n1, d1, (y_ref, xb_ref, fb_ref) = cpu_rate(1, False, a.cpu_seconds / 3)
+        rA, nA, dA, _ = cpu_rate(threads, False, a.cpu_seconds / 3)
+        rS, nS, dS, _ = cpu_rate(threads, True, a.cpu_seconds / 3)
+        cpu = dict(value=round(rA, 4), unit='Mpix/s', cores=threads, kind='port',
+                   sample=f'{nA} x (fwd + x-adj + f-adj) of one 1024^2 frame, FP64 NumPy oracle with scipy.fft '
+                          f'on {threads} threads, {dA:.2f} s each ({where})',
+                   variants=[dict(value=round(r1, 4), cores=1, precision='fp64',
+                                  sample=f'{n1} steps, numpy.fft single thread, {d1:.2f} s each'),
+                             dict(value=round(rA, 4), cores=threads, precision='fp64',
+                                  sample=f'{nA} steps, scipy.fft workers={threads}, {dA:.2f} s each'),
+                             dict(value=round(rS, 4), cores=threads, precision='fp32 FFTs and products',
+                                  sample=f'{nS} steps, scipy.fft complex64 workers={threads}, {dS:.2f} s each')])
         if graph is not None:
             graph.replay()
             torch.cuda.synchronize()
@@ -388,7 +465,6 @@ def main():
         'ms_per_frame': round(ms / B, 4),
         'eager_batched_ms_per_step': round(eager_ms, 4),
         'single_frame_eager': {'ms': round(ms1, 4), 'mpix_s': round(H * W / (ms1 * 1e-3) / 1e6, 2)},
-        'hbm_frac_step': round(step_hbm / HBM_PEAK_GBS, 5),
         'step_alg_bytes': op_bytes,
         'roofline': roofline,
         'kernels': kernels,

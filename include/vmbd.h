@@ -79,6 +79,11 @@ int vmbd_version(void);
 
 /* Message of the last failed call on this thread ("" if none). */
 const char* vmbd_last_error(void);
+/* Status code (VMBD_E*) of the last failed call on this thread, VMBD_OK if
+ * the last call succeeded; for vmbd_plan_create, which returns NULL on
+ * failure (no counterpart in the reference, whose errors are Python
+ * exceptions, olaGPU.py:104-136). */
+int vmbd_last_status(void);
 
 /* Create a plan on the current HIP device.
  * win_table: host array (n_win, sw0, sw1) of float32 windows ws_k;

@@ -33,7 +33,7 @@ def _filter_spectrum(taps, T):
     g = np.zeros(T, dtype=np.complex128)
     taps = np.asarray(taps, dtype=np.float64)
     g[:taps.shape[0], :taps.shape[1]] = taps
-    return np.fft.fft2(g)
+    return O.fft2(g)
 
 
 def _offset(Z):
@@ -57,8 +57,8 @@ def xdirect(Z, y, alpha, beta, edgetapering=1):
     off = _offset(Z)
     yp = O.chop_pad(np.asarray(y, dtype=np.float64), wa.csf, wa.sw, wa.nhop, T, off)
     ws = O.pad_stack(wa.ws, T)
-    yp = np.fft.fft2(ws * yp)
-    z = np.real(np.fft.ifft2(fourier_div(Z.spec, yp, L, alpha, beta)))
+    yp = O.fft2(ws * yp)
+    z = np.real(O.ifft2(fourier_div(Z.spec, yp, L, alpha, beta)))
     if Z.id == 'X':
         return O.crop_stack(z, Z.sf)
     zs = O.crop_stack(z, wa.sw)
@@ -98,11 +98,11 @@ def gdirect(Z, y, alpha, beta, edgetapering=1):
     xx, xy = FO.gradient(Z.x)
     yx, yy = FO.gradient(y)
     off = _offset(Z)
-    yx = np.fft.fft2(O.chop_pad(yx, wa.csf, wa.sw, wa.nhop, T, off))
-    yy = np.fft.fft2(O.chop_pad(yy, wa.csf, wa.sw, wa.nhop, T, off))
-    xx = np.fft.fft2(O.chop_pad(xx, wa.csf, wa.sw, wa.nhop, T))
-    xy = np.fft.fft2(O.chop_pad(xy, wa.csf, wa.sw, wa.nhop, T))
-    z = np.real(np.fft.ifft2(fourier_div_g(xx, xy, yx, yy, L, alpha, beta)))
+    yx = O.fft2(O.chop_pad(yx, wa.csf, wa.sw, wa.nhop, T, off))
+    yy = O.fft2(O.chop_pad(yy, wa.csf, wa.sw, wa.nhop, T, off))
+    xx = O.fft2(O.chop_pad(xx, wa.csf, wa.sw, wa.nhop, T))
+    xy = O.fft2(O.chop_pad(xy, wa.csf, wa.sw, wa.nhop, T))
+    z = np.real(O.ifft2(fourier_div_g(xx, xy, yx, yy, L, alpha, beta)))
     return O.crop_stack(z, Z.sf)
 
 
@@ -125,20 +125,54 @@ def sparse(Z, y, alpha, gamma=1e-4, edgetapering=1):
     ws = O.pad_stack(wa.ws, T, off)
     yp = O.chop_pad(y, wa.csf, wa.sw, wa.nhop, T, off) * ws
     cp = O.chop_pad(const, wa.csf, wa.sw, wa.nhop, T, off) * ws
-    cp = np.real(np.fft.ifft2(fourier_div(Z.spec, np.fft.fft2(cp), L, alpha, gamma)))
+    cp = np.real(O.ifft2(fourier_div(Z.spec, O.fft2(cp), L, alpha, gamma)))
     cs = O.crop_stack(cp, wa.sw) * wa.ws
     const = O.crop(O.ola(cs, wa.csf, wa.sw, wa.nhop), Z.sx)
     const = 0.01 / np.maximum(const, 0.01)
-    ftpy = np.fft.fft2(yp) * np.conj(Z.spec)
+    ftpy = O.fft2(yp) * np.conj(Z.spec)
     while beta < beta_max:
         xx, xy = FO.gradient(x)
         xx = FO.shrink23(xx, beta)
         xy = FO.shrink23(xy, beta)
-        wx = np.fft.fft2(O.chop_pad(xx, wa.csf, wa.sw, wa.nhop, T))
-        wy = np.fft.fft2(O.chop_pad(xy, wa.csf, wa.sw, wa.nhop, T))
-        z = np.real(np.fft.ifft2(fourier_div_s(gx, gy, wx, wy, ftpy, Z.spec, L, alpha, beta, gamma)))
+        wx = O.fft2(O.chop_pad(xx, wa.csf, wa.sw, wa.nhop, T))
+        wy = O.fft2(O.chop_pad(xy, wa.csf, wa.sw, wa.nhop, T))
+        z = np.real(O.ifft2(fourier_div_s(gx, gy, wx, wy, ftpy, Z.spec, L, alpha, beta, gamma)))
         zs = O.crop_stack(z, wa.sw) * wa.ws
         x = O.crop(O.ola(zs, wa.csf, wa.sw, wa.nhop), Z.sx) * const
         x = np.maximum(x, 0.0)
         beta *= beta_rate
     return x
+
+
+def cnvinv_objfun(Z, z, y, alpha=0.0, beta=0.0):
+    """olaGPU.py:756-786: 1/2 |y - Z z|^2 (+ 1/2 alpha z.Lz with L the
+    'same' 5-point Laplacian of an image z, + 1/2 beta |z|^2).  An 'X'
+    instance with alpha > 0 calls an undefined module in the reference
+    (olaGPU.py:775): NameError here too."""
+    z = np.asarray(z, dtype=np.float64)
+    r = np.asarray(y, dtype=np.float64) - Z.cnv(z)
+    obj = 0.5 * np.sum(r * r)
+    if alpha > 0:
+        if Z.id == 'X':
+            raise NameError("name 'shock' is not defined")
+        obj += 0.5 * alpha * np.sum(z * FO.laplace(z, 'same'))
+    if beta > 0:
+        obj += 0.5 * beta * np.sum(z * z)
+    return obj
+
+
+def cnvinv_gradfun(Z, z, y, alpha=0.0, beta=0.0, exact=False):
+    """olaGPU.py:788-811: -(Z'(y - Z z) + alpha L z + beta z), the sign the
+    reference returns: the data term enters as the true gradient, the two
+    Tikhonov terms with the opposite sign.  Z' is the patch-local transpose
+    (exact=False, OlaGPU's) or the true adjoint."""
+    z = np.asarray(z, dtype=np.float64)
+    r = np.asarray(y, dtype=np.float64) - Z.cnv(z)
+    g = Z.cnvtp(r, exact=exact)
+    if alpha > 0:
+        if Z.id == 'X':
+            raise NameError("name 'shock' is not defined")
+        g = g + alpha * FO.laplace(z, 'same')
+    if beta > 0:
+        g = g + beta * z
+    return -g

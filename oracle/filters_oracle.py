@@ -15,6 +15,14 @@ SQRT2H = np.sqrt(0.5)
 def shock_step(a, dt=0.1, h=1.0):
     """kernel.cu:9-88, one step.  Neighbours above/left of the image replicate
     the edge, neighbours below/right read 0 (kernel.cu:22-30)."""
+    c, flow, Inn = shock_terms(a)
+    return c - dt * np.copysign(flow, Inn) / h
+
+
+def shock_terms(a):
+    """The pixel values c, the minmod gradient magnitude |grad I| and the
+    second directional derivative Inn of one shock step (kernel.cu:9-88);
+    the step is c - dt sign(Inn) |grad I| / h."""
     a = np.asarray(a, dtype=np.float64)
     H, W = a.shape
     p = np.zeros((H + 2, W + 2))
@@ -39,7 +47,7 @@ def shock_step(a, dt=0.1, h=1.0):
     Ixy = (p[2:H + 2, 2:W + 2] - p[2:H + 2, 0:W] - p[0:H, 2:W + 2] + p[0:H, 0:W]) / 4
     Inn = (Iyy * Iy * Iy + 2 * Ixy * Iy * Ix + Ixx * Ix * Ix) / (Ix * Ix + Iy * Iy + 1e-8)
     Inn = np.where((Ix == 0) & (Iy == 0), Ixx, Inn)
-    return c - dt * np.copysign(flow, Inn) / h
+    return c, flow, Inn
 
 
 def shock_filter(a, iters=3, dt=0.1, h=1.0):

@@ -247,12 +247,39 @@ def fft_tile(sw, sf):
     return tuple(int(v) for v in 2 ** np.ceil(np.log2(s)))
 
 
+# FFT execution settings (the same FP64 DFT either way).  WORKERS = 1 runs
+# numpy's pocketfft on one thread; WORKERS > 1 runs scipy.fft's pocketfft on
+# that many threads (the large parity cases and the all-cores CPU baseline).
+# SINGLE = True runs the transforms, and with them the spectral products, in
+# complex64 (the FP32 CPU-baseline variant; overlap-add sums stay FP64).
+WORKERS = 1
+SINGLE = False
+
+
+def set_fft(workers=1, single=False):
+    """Select the FFT thread count and precision; returns the old setting."""
+    global WORKERS, SINGLE
+    old = (WORKERS, SINGLE)
+    WORKERS, SINGLE = max(1, int(workers)), bool(single)
+    return old
+
+
+def _fft(a, inverse):
+    if SINGLE:
+        a = np.asarray(a, dtype=np.complex64)
+    if WORKERS > 1 or SINGLE:
+        import scipy.fft as sf
+        fn = sf.ifft2 if inverse else sf.fft2
+        return fn(a, axes=(-2, -1), workers=WORKERS)
+    return (np.fft.ifft2 if inverse else np.fft.fft2)(a, axes=(-2, -1))
+
+
 def fft2(a):
-    return np.fft.fft2(a, axes=(-2, -1))
+    return _fft(a, False)
 
 
 def ifft2(a):
-    return np.fft.ifft2(a, axes=(-2, -1))
+    return _fft(a, True)
 
 
 class OlaOracle:

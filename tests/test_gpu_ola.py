@@ -211,6 +211,40 @@ def test_row_reuse(cuda):
     assert rel_l2(rf.cpu().numpy(), O.adjoint_f(np.float32(g).astype(np.float64), x2, owa, sf, exact=True)) <= TOL
 
 
+@pytest.mark.parametrize('writer', ['cliplower', 'clip', 'edgetaper', 'out'])
+def test_row_reuse_after_library_write(writer, cuda):
+    """forward(x, f), then a library kernel rewrites x in place through a raw
+    pointer, then adjoint_xf(g, x, f): the adjoint must see the new x (the
+    wrappers move torch's version counter), checked against the oracle
+    evaluated on the modified x."""
+    import torch
+    from vmbd_amd import gputools
+    sx, csf, sf = (256, 256), (4, 4), (15, 15)
+    print(f'case reuse after {writer}', flush=True)
+    wa, plan, owa = _mk(sx, csf, 0.5, sf, 'valid', cuda)
+    x, f, g = case_inputs(32, sx, csf, sf)
+    xt, ft, gt = _t(x, cuda), _t(f, cuda), _t(g, cuda)
+    plan.forward(xt, ft)
+    if writer == 'cliplower':
+        gputools.cliplower_GPU(xt, 0.3)
+    elif writer == 'clip':
+        gputools.clip_GPU(xt, 0.2, 0.7)
+    elif writer == 'edgetaper':
+        gputools.edgetaper_gpu(xt, (31, 31))
+    else:
+        # another plan writes x as its output buffer
+        wa2, plan2, _ = _mk(sx, csf, 0.5, sf, 'same', cuda)
+        plan2.forward(_t(case_inputs(33, sx, csf, sf)[0], cuda), ft, out=xt)
+    xmod = xt.cpu().numpy().astype(np.float64)
+    assert not np.array_equal(xmod, np.float32(x).astype(np.float64))
+    xb, fb = plan.adjoint_xf(gt, xt, ft)
+    g32 = np.float32(g).astype(np.float64)
+    f32 = np.float32(f).astype(np.float64)
+    assert rel_l2(fb.cpu().numpy(), O.adjoint_f(g32, xmod, owa, sf, exact=True)) <= TOL
+    assert rel_l2(xb.cpu().numpy(), O.adjoint_x(g32, f32, owa, sx, exact=True)) <= TOL
+    torch.cuda.synchronize()
+
+
 def test_graph_capture_replay(cuda):
     """The forward + both transposes captured in a HIP graph replay to the
     eager results, with eager work between replays."""

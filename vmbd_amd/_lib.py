@@ -37,6 +37,7 @@ _I = ctypes.c_int32
 SIGNATURES = {
     'vmbd_version': (ctypes.c_int, []),
     'vmbd_last_error': (ctypes.c_char_p, []),
+    'vmbd_last_status': (ctypes.c_int, []),
     'vmbd_plan_create': (_P, [ctypes.POINTER(VmbdGeom), ctypes.c_void_p, _I, ctypes.c_void_p]),
     'vmbd_plan_destroy': (None, [_P]),
     'vmbd_plan_workspace_bytes': (ctypes.c_size_t, [_P]),
@@ -119,14 +120,24 @@ def lib():
     return _lib
 
 
+VMBD_E_ARG = -1
+VMBD_E_HIP = -2
+VMBD_E_NOMEM = -3
+VMBD_E_UNSUP = -4
+
+
 class VmbdError(RuntimeError):
-    pass
+    """A failed library call; ``code`` is its VMBD_E* status."""
+
+    def __init__(self, msg, code=None):
+        super().__init__(msg)
+        self.code = code
 
 
 def check(rc, what=''):
     if rc != VMBD_OK:
         msg = lib().vmbd_last_error().decode(errors='replace')
-        raise VmbdError(f'{what}: vmbd error {rc}: {msg}')
+        raise VmbdError(f'{what}: vmbd error {rc}: {msg}', rc)
 
 
 def call(name, *args):
@@ -136,6 +147,17 @@ def call(name, *args):
 
 def last_error():
     return lib().vmbd_last_error().decode(errors='replace')
+
+
+def wrote(*tensors):
+    """Record that the library has written these tensors through raw
+    pointers: torch's version counter of each is bumped, as a torch in-place
+    op would, so that OlaPlan's row-transform reuse (keyed on the version)
+    never trusts a transform of the old contents.  None entries and
+    inference tensors (which carry no counter) are skipped."""
+    for t in tensors:
+        if isinstance(t, torch.Tensor) and not t.is_inference():
+            torch.autograd.graph.increment_version(t)
 
 
 def ptr(t):

@@ -18,8 +18,14 @@ from .imagetools import fft_tile
 CACHE = os.path.join(os.path.expanduser('~'), '.cache', 'vmbd_amd', 'tiles.json')
 
 
-def _key(sx, sf, csf, overlap, frames):
-    return f'{sx[0]}x{sx[1]}/{sf[0]}x{sf[1]}/{csf[0]}x{csf[1]}/{overlap}/{frames}'
+def _key(sx, sf, csf, overlap, frames, device):
+    """Cache key: the geometry, the frame count, the device's architecture
+    and the library version (a timing taken on another GPU or with other
+    kernels is not reused)."""
+    from . import _lib
+    arch = torch.cuda.get_device_properties(device).gcnArchName
+    ver = _lib.lib().vmbd_version()
+    return f'{sx[0]}x{sx[1]}/{sf[0]}x{sf[1]}/{csf[0]}x{csf[1]}/{overlap}/{frames}/{arch}/v{ver}'
 
 
 def candidates(sw, sf):
@@ -50,11 +56,29 @@ def _store(d):
         pass
 
 
-def time_tile(sx, sf, winaux, tile, frames=1, reps=5, device=None):
-    """Milliseconds of forward + adjoint_xf of `frames` frames on `tile`."""
+def make_plan(sx, sf, winaux, tile, frames=1, device=None):
+    """The plan of a candidate tile, or None when the library refuses the
+    tile for this geometry (VMBD_E_UNSUP / VMBD_E_ARG at plan creation, e.g.
+    an overlap-add that needs more LDS than a workgroup has); any other
+    failure propagates."""
+    from . import _lib
     from .ola import OlaPlan
     dev = torch.device(device) if device is not None else torch.device('cuda', torch.cuda.current_device())
-    plan = OlaPlan(sx, sf, winaux, 'valid', tile=tile, device=dev, max_frames=frames)
+    try:
+        return OlaPlan(sx, sf, winaux, 'valid', tile=tile, device=dev, max_frames=frames)
+    except _lib.VmbdError as e:
+        if e.code in (_lib.VMBD_E_UNSUP, _lib.VMBD_E_ARG):
+            return None
+        raise
+
+
+def time_tile(sx, sf, winaux, tile, frames=1, reps=5, device=None, plan=None):
+    """Milliseconds of forward + adjoint_xf of `frames` frames on `tile`."""
+    dev = torch.device(device) if device is not None else torch.device('cuda', torch.cuda.current_device())
+    if plan is None:
+        plan = make_plan(sx, sf, winaux, tile, frames, dev)
+        if plan is None:
+            raise ValueError(f'tile {tile} is not supported for this geometry')
     g = torch.Generator(device=dev).manual_seed(0)
     x = torch.rand((frames,) + tuple(sx), generator=g, device=dev)
     f = torch.rand((frames, plan.P) + tuple(sf), generator=g, device=dev)
@@ -82,17 +106,19 @@ def best_tile(sx, sf, winaux, frames=1, overlap=None, use_cache=True, device=Non
     """Fastest tile for the geometry (cached); returns (tile, {tile: ms})."""
     csf = winaux.csf
     ov = overlap if overlap is not None else getattr(winaux, 'overlap', 0.5)
-    key = _key(sx, sf, csf, ov, frames)
+    dev = torch.device(device) if device is not None else torch.device('cuda', torch.cuda.current_device())
+    key = _key(sx, sf, csf, ov, frames, dev)
     cache = _load() if use_cache else {}
     if key in cache:
         return tuple(cache[key]['tile']), {tuple(map(int, k.split('x'))): v for k, v in cache[key]['ms'].items()}
     sw = np.broadcast_to(np.asarray(winaux.sw), (2,))
     times = {}
     for t in candidates(sw, sf):
-        try:
-            times[t] = time_tile(sx, sf, winaux, t, frames, device=device)
-        except (RuntimeError, ValueError):
+        plan = make_plan(sx, sf, winaux, t, frames, dev)
+        if plan is None:
             continue            # a tile the plan rejects (LDS limits) is not a candidate
+        times[t] = time_tile(sx, sf, winaux, t, frames, device=dev, plan=plan)
+        del plan
     if not times:
         raise RuntimeError('no tile candidate is supported for this geometry')
     best = min(times, key=times.get)

@@ -260,7 +260,7 @@ Geo call_geo(const vmbd_plan* p, int nframes) {
 
 extern "C" {
 
-int vmbd_version(void) { return 100; }
+int vmbd_version(void) { return 200; }
 
 vmbd_plan* vmbd_plan_create(const vmbd_geom* gm, const float* win_table, int32_t n_win, const int32_t* win_idx) {
   vmbd_clear_error();
@@ -340,10 +340,9 @@ vmbd_plan* vmbd_plan_create(const vmbd_geom* gm, const float* win_table, int32_t
     vmbd_fail(VMBD_E_ARG, "vmbd_plan_create: max_frames=%d outside [0, 65535]", G.max_frames);
     return nullptr;
   }
-  if ((long long)(G.max_frames > 0 ? G.max_frames : 1) * G.csf0 * G.csf1 > 65535) {
-    // the column pass puts (frame, patch) on the second grid dimension
-    vmbd_fail(VMBD_E_ARG, "vmbd_plan_create: max_frames*csf0*csf1=%lld exceeds 65535",
-              (long long)(G.max_frames > 0 ? G.max_frames : 1) * G.csf0 * G.csf1);
+  if ((long long)G.csf0 * G.csf1 > 65535) {
+    // the column pass puts the patch on the second grid dimension (the frame on the third)
+    vmbd_fail(VMBD_E_ARG, "vmbd_plan_create: csf0*csf1=%lld patches exceed 65535", (long long)G.csf0 * G.csf1);
     return nullptr;
   }
   for (int k = 0; k < g.Pf; ++k) {

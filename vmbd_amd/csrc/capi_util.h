@@ -7,12 +7,18 @@
 
 #include "../../include/vmbd.h"
 
-// thread-local message buffer behind vmbd_last_error()
+// thread-local message buffer behind vmbd_last_error() and the status code
+// behind vmbd_last_status()
 char* vmbd_err_buf();
+int& vmbd_err_code();
 
-inline void vmbd_clear_error() { vmbd_err_buf()[0] = '\0'; }
+inline void vmbd_clear_error() {
+  vmbd_err_buf()[0] = '\0';
+  vmbd_err_code() = VMBD_OK;
+}
 
 inline int vmbd_fail(int code, const char* fmt, ...) {
+  vmbd_err_code() = code;
   va_list ap;
   va_start(ap, fmt);
   vsnprintf(vmbd_err_buf(), 512, fmt, ap);

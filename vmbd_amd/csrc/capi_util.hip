@@ -6,4 +6,11 @@ char* vmbd_err_buf() {
   return buf;
 }
 
+int& vmbd_err_code() {
+  static thread_local int code = VMBD_OK;
+  return code;
+}
+
 extern "C" const char* vmbd_last_error(void) { return vmbd_err_buf(); }
+
+extern "C" int vmbd_last_status(void) { return vmbd_err_code(); }

@@ -400,7 +400,7 @@ __global__ __launch_bounds__(WG, (TH >= 256 ? 4 : 1)) void k_cols(Geo g, ColIn a
   // strips of one patch are consecutive workgroups; each writes whole
   // 128-byte lines of its rows (16 bins, row pitch NBP a multiple of 16)
   const int strip = blockIdx.x;
-  const int p = blockIdx.y;
+  const int p = (int)blockIdx.z * g.Pf + (int)blockIdx.y;  // frame blockIdx.z, patch blockIdx.y
   const int cc = threadIdx.x / TPT;
   const int t = threadIdx.x - cc * TPT;
   const int c = strip * CS + cc;
@@ -495,7 +495,7 @@ static hipError_t cols_t(const Geo& g, int op, const ColIn& a, const ColIn& b, c
                          const ColOut& o2, float2* spec_out, const float2* tw, hipStream_t st) {
   constexpr int CS = WG / (TH / 16);
   const int strips = (g.NB - 1 + CS - 1) / CS;  // bins [0, NB-1): NB-1 is packed into bin 0
-  const dim3 grid((unsigned)strips, (unsigned)g.P);
+  const dim3 grid((unsigned)strips, (unsigned)g.Pf, (unsigned)(g.P / g.Pf));
   static const char* names[4] = {"cols_spec", "cols_fwd", "cols_adj1", "cols_adj2"};
   ProfScope ps(names[op & 3], st);
   switch (op) {

@@ -35,10 +35,13 @@ def _p(t):
 
 def _inplace(t, name):
     """Check that an in-place operand is a contiguous float32 device tensor
-    (the kernels address it as one flat array of numel() elements)."""
+    (the kernels address it as one flat array of numel() elements) and mark
+    it modified (its torch version counter moves, as with a torch in-place
+    op), since the kernel about to run writes it through a raw pointer."""
     if not (isinstance(t, torch.Tensor) and t.device.type == 'cuda' and t.dtype == torch.float32
             and t.is_contiguous()):
         raise TypeError(f'{name}: expected a contiguous float32 device tensor')
+    _lib.wrote(t)
     return t
 
 

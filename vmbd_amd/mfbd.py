@@ -265,6 +265,9 @@ class OnlineMFBD:
             if self._graph is None:
                 self._record()
             self._graph.replay()
+        # the latent image and the PSFs were written through raw pointers
+        # (x-step, prior, normalisation): mark them modified
+        _lib.wrote(self.x, self.f_dev)
         self.frame += 1
         self.f = self.f_dev
         return self.x, self.f_dev

@@ -17,6 +17,10 @@ the true adjoint, ``exact=False`` the reference's patch-local transpose
 ``adjoint_xf`` reuses the row transforms of x and f computed by the
 preceding ``forward`` when it is given the very same tensors (same storage,
 unmodified since, same stream, nothing else run on the plan in between).
+"Unmodified" is torch's version counter: torch in-place ops move it, and
+so does every library wrapper that writes a tensor through a raw pointer
+(``_lib.wrote``).  A write that bypasses both (a raw pointer of the
+caller's own) must pass ``reuse=False``.
 """
 import ctypes
 
@@ -63,7 +67,7 @@ class OlaPlan:
             h = L.vmbd_plan_create(ctypes.byref(g), table.ctypes.data_as(ctypes.c_void_p), int(table.shape[0]),
                                    idx.ctypes.data_as(ctypes.c_void_p))
         if not h:
-            raise _lib.VmbdError(f'vmbd_plan_create: {_lib.last_error()}')
+            raise _lib.VmbdError(f'vmbd_plan_create: {_lib.last_error()}', L.vmbd_last_status())
         self._h = ctypes.c_void_p(h)
         a, b = ctypes.c_int32(), ctypes.c_int32()
         _lib.check(L.vmbd_plan_out_shape(self._h, ctypes.byref(a), ctypes.byref(b)), 'out_shape')
@@ -129,6 +133,7 @@ class OlaPlan:
         if (tuple(out.shape) != tuple(shape) or out.dtype != torch.float32 or not out.is_contiguous()
                 or out.device != self.device):
             raise ValueError('out has the wrong shape/dtype/layout/device')
+        _lib.wrote(out)  # the kernels write it through a raw pointer
         return out
 
     @staticmethod
@@ -137,12 +142,15 @@ class OlaPlan:
 
     def _key(self, x, f, nf):
         # the tensors themselves are held (not their addresses), so a freed
-        # and reallocated buffer can never match a stale key
+        # and reallocated buffer can never match a stale key; inference
+        # tensors carry no version counter and are never reused
+        if x.is_inference() or f.is_inference():
+            return None
         return (x, x._version, f, f._version, nf, torch.cuda.current_stream().cuda_stream)
 
     def _key_matches(self, x, f, nf):
         k = self._rows_key
-        if k is None:
+        if k is None or x.is_inference() or f.is_inference():
             return False
         kx, vx, kf, vf, knf, kst = k
         return (kx is x and kf is f and vx == x._version and vf == f._version and knf == nf

@@ -125,6 +125,7 @@ def pbb_step(x, oldx, g, oldg, it, out, scratch, unconstrained=False):
               _lib.ptr(scratch.partials), None, _lib.stream_ptr())
     _lib.call('vmbd_pbb_update', _lib.ptr(out), _lib.ptr(x), _lib.ptr(g), n, _lib.ptr(scratch.partials), int(it),
               int(unconstrained), _lib.ptr(scratch.step), _lib.stream_ptr())
+    _lib.wrote(out)
     return out
 
 
