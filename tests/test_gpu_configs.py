@@ -117,15 +117,20 @@ def test_streamed_rounds_tile512(cuda):
 
 
 def _check_shock(got, a, dt):
-    """One shock step against the oracle: pixels may differ only where the
-    FP64 second directional derivative is zero to within FP32 rounding of
-    its terms (the step's sign decision is a tie there)."""
+    """One shock step against the oracle.  A pixel may differ only where the
+    step's sign decision is a tie: the FP64 second directional derivative
+    Inn is within 1e-6 of the image's range of zero (in the near-flat
+    regions of a smoothed image its FP32 value is cancellation noise: both
+    gradient components sit at the rounding level of the pixel values).
+    Such pixels must be rare, and the rest agree to 1e-5."""
     c, flow, Inn = FO.shock_terms(a)
     ref = c - dt * np.copysign(flow, Inn)
     bad = np.abs(got - ref) > 1e-5 * (1 + np.abs(ref))
     if bad.any():
-        assert np.abs(Inn[bad]).max() <= 1e-4 * np.abs(a).max(), np.abs(Inn[bad]).max()
-    assert bad.sum() <= 40
+        assert np.abs(Inn[bad]).max() <= 1e-6 * np.abs(a).max(), np.abs(Inn[bad]).max()
+    print(f'shock step: {int(bad.sum())} sign ties, max |Inn| there '
+          f'{np.abs(Inn[bad]).max() if bad.any() else 0:.2e}', flush=True)
+    assert bad.mean() < 1e-3
     assert rel_l2(got[~bad], ref[~bad]) <= TOL
     return int(bad.sum())
 
