@@ -60,6 +60,20 @@ int vmbd_pbb_update(float* xnew, const float* x, const float* g, int64_t n, cons
 int vmbd_pbb_step(float* xnew, const float* x, const float* oldx, const float* g, const float* oldg, int64_t n,
                   int32_t iter, int32_t unconstrained, double* partials, float* step_out, vmbd_stream_t stream);
 
+/* vmbd_pbb_step with its reduction scratch in a static device array of the
+ * library instead of a caller's `partials` buffer: it allocates nothing
+ * (no hipMalloc, no caller workspace).  No counterpart in the reference; it
+ * is for callers that cannot allocate: code under HIP graph capture, tight
+ * solver loops, harnesses that forbid allocation.  Limit: the scratch holds
+ * 3 * VMBD_MAX_PARTIALS doubles, the partial sums of a step over any n (the
+ * reductions use at most VMBD_MAX_PARTIALS blocks); larger needs are
+ * refused with VMBD_E_UNSUP.  Each GPU holds one copy until the process
+ * ends, and every call on that GPU shares it: calls that overlap on one GPU
+ * (other streams, other host threads) race and must be ordered by the
+ * caller. */
+int vmbd_pbb_step_noalloc(float* xnew, const float* x, const float* oldx, const float* g, const float* oldg,
+                          int64_t n, int32_t iter, int32_t unconstrained, float* step_out, vmbd_stream_t stream);
+
 /* Start pair of PBB (optGPU.py:256-297, replaces the host comparison of
  * the two start objectives): dots = device [r.r, z.z] at the first point and
  * at the second, objective r.r/2 + beta z.z/2; when the second point is not

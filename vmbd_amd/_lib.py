@@ -83,6 +83,7 @@ SIGNATURES = {
     'vmbd_ew': (ctypes.c_int, [_F, _F, _F, ctypes.c_float, ctypes.c_int64, _I, _P]),
     'vmbd_pbb_step': (ctypes.c_int, [_F, _F, _F, _F, _F, ctypes.c_int64, _I, _I, _P, _F, _P]),
     'vmbd_pbb_select': (ctypes.c_int, [_F, _F, _F, _F, ctypes.c_int64, _P, ctypes.c_double, _P]),
+    'vmbd_pbb_step_noalloc': (ctypes.c_int, [_F, _F, _F, _F, _F, ctypes.c_int64, _I, _I, _F, _P]),
     'vmbd_tv': (ctypes.c_int, [_F, _I, _I, _I, ctypes.c_float, _P, _P, _P]),
     'vmbd_tv_grad': (ctypes.c_int, [_F, _F, _I, _I, _I, ctypes.c_float, _P, ctypes.c_float, _I, _P]),
     'vmbd_rl_update': (ctypes.c_int, [_F, _F, _F, _I, _I, ctypes.c_float, _I, _I, _F, _F, _P]),

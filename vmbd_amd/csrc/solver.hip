@@ -10,6 +10,7 @@
 
 #include <cstdint>
 #include <initializer_list>
+#include <mutex>
 
 #include "../../include/vmbd_solver.h"
 #include "capi_util.h"
@@ -362,6 +363,41 @@ __global__ __launch_bounds__(TB) void k_tv_grad(float* grad, const float* x, lon
 }
 
 }  // namespace
+
+// Scratch of vmbd_pbb_step_noalloc: the partial sums of one PBB step
+// (3 per reduction block, at most VMBD_MAX_PARTIALS blocks), held by every
+// GPU the library runs on until the process ends.  Host code never uses the
+// array's name as an address: its device address comes from
+// hipGetSymbolAddress, once per device (noalloc_scratch).
+constexpr int NOALLOC_DOUBLES = 3 * VMBD_MAX_PARTIALS;
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
+namespace {
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// device address of this device's scratch (looked up on its first use)
+int noalloc_scratch(double** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return vmbd_fail_hip("vmbd_pbb_step_noalloc: hipGetDevice", e);
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES)
+    return vmbd_fail(VMBD_E_UNSUP, "vmbd_pbb_step_noalloc: device %d beyond the %d supported", dev,
+                     NOALLOC_MAX_DEVICES);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return vmbd_fail_hip("vmbd_pbb_step_noalloc: hipGetSymbolAddress", e);
+    if (!p) return vmbd_fail(VMBD_E_HIP, "vmbd_pbb_step_noalloc: no device address for the scratch");
+    g_scratch_ptr[dev] = (double*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return VMBD_OK;
+}
+
+}  // namespace
 }  // namespace vmbd
 
 using namespace vmbd;
@@ -550,3 +586,19 @@ int vmbd_rl_update(float* x, const float* num, const float* den, int32_t H, int3
 }
 
 }  // extern "C"
+
+extern "C" int vmbd_pbb_step_noalloc(float* xnew, const float* x, const float* oldx, const float* g,
+                                     const float* oldg, int64_t n, int32_t iter, int32_t unconstrained,
+                                     float* step_out, vmbd_stream_t stream) {
+  vmbd_clear_error();
+  if (!xnew || !x || !oldx || !g || !oldg || n <= 0)
+    return vmbd_fail(VMBD_E_ARG, "vmbd_pbb_step_noalloc: bad argument");
+  // the partials of any n fit: the reductions use at most VMBD_MAX_PARTIALS blocks
+  if (3 * grid_for(n) > NOALLOC_DOUBLES)
+    return vmbd_fail(VMBD_E_UNSUP, "vmbd_pbb_step_noalloc: %d partial sums exceed the %d-double scratch",
+                     3 * grid_for(n), NOALLOC_DOUBLES);
+  double* partials = nullptr;
+  if (int r = noalloc_scratch(&partials)) return r;
+  // every partial the update reads is written by the statistics kernel first
+  return vmbd_pbb_step(xnew, x, oldx, g, oldg, n, iter, unconstrained, partials, step_out, stream);
+}
