@@ -116,26 +116,27 @@ def attainable_hbm_frac():
 
 
 def operator_roofline(kernels, B, ms):
-    """Roofline of the dominant kernel on the operator's algorithmic bytes and
-    flops (one fwd + x-adj + f-adj per frame, inputs and outputs once,
-    windows generated in the kernels): the B frames' bytes and flops over
-    the kernel's average launch time, and over the whole step's time.  The
-    step is VALU-issue bound (52 flop per byte against a ridge of 10 for
-    non-packed FP32), so the HBM fraction it can reach is capped at
-    attainable_hbm_frac()."""
+    """Roofline on the operator's algorithmic bytes and flops (one fwd +
+    x-adj + f-adj per frame, inputs and outputs once, windows generated in
+    the kernels): `achieved`/`frac` are the B frames' bytes over the dominant
+    kernel's average launch time (the intermediate spectra it moves are not
+    algorithmic bytes; they are under `kernels`), `step` the same bytes and
+    the flops over the whole step's time.  The step is VALU-issue bound (52
+    flop per byte against a ridge of 10 for non-packed FP32), so `valu`
+    gives its flop rate against the non-packed FP32 peak, and the HBM
+    fraction it can reach is capped at attainable_hbm_frac()."""
     dom = max(kernels, key=lambda k: kernels[k]['total_us_per_step'])
     d = kernels[dom]
     by, fl = operator_bytes() * B, operator_flops() * B
-    t_k = d['avg_us'] * 1e-6
-    gbs = by / t_k / 1e9
-    tf = fl / t_k / 1e12
+    gbs = by / (d['avg_us'] * 1e-6) / 1e9
     step_gbs = by / (ms * 1e-3) / 1e9
     step_tf = fl / (ms * 1e-3) / 1e12
     return dict(kernel=dom, bound='valu', achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit='GB/s',
                 frac=round(gbs / HBM_PEAK_GBS, 4), traffic=measured_traffic(dom, B), avg_us=d['avg_us'],
-                alg_bytes_per_launch=by, alg_flops_per_launch=fl,
-                valu=dict(achieved_tflops=round(tf, 2), peak_nonpacked=VALU_FP32_PEAK,
-                          frac=round(tf / VALU_FP32_PEAK, 4)),
+                alg_bytes_per_launch=by,
+                # the bound: algorithmic flops of the step over the step's time
+                valu=dict(basis='step', alg_flops=fl, achieved_tflops=round(step_tf, 2),
+                          peak_nonpacked=VALU_FP32_PEAK, frac=round(step_tf / VALU_FP32_PEAK, 4)),
                 step=dict(hbm_gbs=round(step_gbs, 1), hbm_frac=round(step_gbs / HBM_PEAK_GBS, 5),
                           valu_tflops=round(step_tf, 2), valu_frac=round(step_tf / VALU_FP32_PEAK, 4)),
                 attainable_hbm_frac=round(attainable_hbm_frac(), 4))
