@@ -167,6 +167,45 @@ int vmbd_ola_forward_cached_ex(vmbd_plan* plan, int32_t nframes, int32_t kind, c
 int vmbd_ola_adjoint_cached_ex(vmbd_plan* plan, int32_t nframes, int32_t kind, const void* spec, const float* g,
                                const float* z, float beta, float* out, int32_t exact, vmbd_stream_t stream);
 
+/* Closed-form deconvolutions of OlaGPU.deconv on the plan (one frame): the
+ * observation's patches are transformed by the row pass, the per-patch
+ * Fourier division (gputools.cu:316-406) is formed on the half spectra in the
+ * column pass with the filter energies in closed form, and the inverse rows
+ * go straight into the output: three launches per division instead of the
+ * reference's chop / pad / FFT / divide / IFFT / crop / OLA sequence.
+ * spec is the plan's cached spectrum (vmbd_spectra) of the instance's
+ * operand; work holds vmbd_deconv_workspace_bytes() bytes of device memory.
+ *
+ * vmbd_deconv_direct  ('xdirect', olaGPU.py:550-609): per patch
+ *   conj(C) Y / (|C|^2 + alpha L + beta), L the energy of the [-1 1] and
+ *   [-1 1]' filters, Y the observation's patch at the mode offset times the
+ *   window at tile offset 0 (the reference's placement).  kind PSF ('F'
+ *   instance): out = image H x W (patches cropped to sw, overlap-added);
+ *   kind IMAGE ('X' instance): out = PSFs P x sfh x sfw.
+ * vmbd_deconv_gdirect ('gdirect', olaGPU.py:416-511, 'X' instance): PSFs from
+ *   the valid-mode gradients xx (H x W-1), xy (H-1 x W) of the latent image
+ *   and yx (sy0 x sy1-1), yy (sy0-1 x sy1) of the (tapered) observation:
+ *   (conj(Xx) Yx + conj(Xy) Yy) / (|Xx|^2 + |Xy|^2 + alpha L + beta), L of
+ *   the 3x2 / 2x3 difference filters.
+ * vmbd_deconv_sparse_setup / _step ('sparse', olaGPU.py:613-733, 'F'
+ *   instance), the half-quadratic splitting: setup forms ftpy = conj(F) Y (a
+ *   spectrum of vmbd_plan_spectrum_elems() complex64) from the tapered
+ *   observation y and cfac = 0.01 / max(xdirect-like deconvolution of the
+ *   tapered constant image c, 0.01) (H x W); each step takes the shrunk
+ *   gradients wx (wx_rows x wx_cols), wy of the current estimate and writes
+ *   x = max(cfac * OLA(windows * IFFT((conj(Gx) Wx + conj(Gy) Wy +
+ *   ftpy/(alpha beta)) / (|F|^2/(alpha beta) + Ls + gamma))), 0). */
+size_t vmbd_deconv_workspace_bytes(const vmbd_plan* plan);
+int vmbd_deconv_direct(vmbd_plan* plan, int32_t kind, const void* spec, const float* y, float alpha, float beta,
+                       float* out, vmbd_stream_t stream);
+int vmbd_deconv_gdirect(vmbd_plan* plan, const float* xx, const float* xy, const float* yx, const float* yy,
+                        float alpha, float beta, float* fout, void* work, vmbd_stream_t stream);
+int vmbd_deconv_sparse_setup(vmbd_plan* plan, const void* spec, const float* y, const float* c, float alpha,
+                             float gamma, void* ftpy, float* cfac, void* work, vmbd_stream_t stream);
+int vmbd_deconv_sparse_step(vmbd_plan* plan, const void* spec, const void* ftpy, const float* wx, int32_t wx_rows,
+                            int32_t wx_cols, const float* wy, int32_t wy_rows, int32_t wy_cols, float alpha,
+                            float beta, float gamma, const float* cfac, float* x, void* work, vmbd_stream_t stream);
+
 /* Opt-in per-kernel timing (no counterpart in the reference, whose only
  * timing is wall-clock prints, stopwatch.py:5-22).  When enabled, every kernel
  * launch is bracketed by HIP events on its stream; vmbd_prof_report()

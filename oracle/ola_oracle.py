@@ -208,6 +208,27 @@ def ola(xs, csf, wm, nhop, offset=(0, 0)):
     return out
 
 
+def ola_flat(xs, csf, wm, nhop):
+    """gputools.cu:295-314 (ola_Kernel_test) as the reference runs it when the
+    extent wm exceeds the slices (F.cnvtp passes sw+sf-1 over sw-sized
+    slices, olaGPU.py:307-309): element (r, c) of patch k is read from the
+    flat stack at k*sh*sw + r*sw + c, so columns past the slice wrap into
+    the next row and rows past it into the next patch.  Reads past the end
+    of the stack (undefined memory in the reference) read 0 here."""
+    P, sh, sw = xs.shape
+    flat = np.asarray(xs, dtype=np.float64).ravel()
+    wm = (int(wm[0]), int(wm[1]))
+    sx = sxp_of(csf, wm, nhop)
+    out = np.zeros((int(sx[0]), int(sx[1])))
+    r = np.arange(wm[0])[:, None]
+    c = np.arange(wm[1])[None, :]
+    for k, (p0, p1) in enumerate(patch_origins(csf, nhop)):
+        idx = k * sh * sw + r * sw + c
+        vals = np.where(idx < flat.size, flat[np.minimum(idx, flat.size - 1)], 0.0)
+        out[p0:p0 + wm[0], p1:p1 + wm[1]] += vals
+    return out
+
+
 # --------------------------------------------------------------------------
 # The operator (olaGPU.py)
 # --------------------------------------------------------------------------
@@ -379,12 +400,13 @@ class OlaOracle:
                     out[k, b0, lo:hi] = y[i, j0 + lo:j0 + hi]
         return out
 
-    def cnvtp(self, y, exact=False):
+    def cnvtp(self, y, exact=False, k1=False):
         """olaGPU.py:261-311: Z = IFFT(G^ . conj(C^)).
 
         'X' -> Re Z[:, :sf, :sf] (PSF-shaped); 'F' -> crop to sw, multiply by
         the windows, OLA with extent sw (the reference passes sw+sf-1 here,
-        reading past each slice; fixed), crop to sx.
+        reading past each slice; fixed), crop to sx.  k1=True reproduces the
+        reference's read past the slices (ola_flat) instead.
         """
         wa = self.winaux
         G = fft2(self.residual_tiles(y, exact))
@@ -392,7 +414,10 @@ class OlaOracle:
         if self.id == 'X':
             return crop_stack(Z, self.sz)
         zs = crop_stack(Z, wa.sw) * wa.ws
-        zc = ola(zs, wa.csf, wa.sw, wa.nhop)
+        if k1:
+            zc = ola_flat(zs, wa.csf, np.asarray(wa.sw) + self.sf - 1, wa.nhop)
+        else:
+            zc = ola(zs, wa.csf, wa.sw, wa.nhop)
         return crop(zc, self.sx)
 
 

@@ -264,12 +264,13 @@ def test_olagpu_deconv_rgb(cuda, small_case):
         np.testing.assert_array_equal(out[..., c], one.astype(np.float32))
 
 
+@pytest.mark.parametrize('fused', [True, False], ids=['fused', 'plain'])
 @pytest.mark.parametrize('mode', ['xdirect', 'direct', 'gdirect', 'sparse'])
-def test_olagpu_closed_form_deconv(cuda, small_case, mode):
+def test_olagpu_closed_form_deconv(cuda, small_case, mode, fused):
     from vmbd_amd.imagetools import win2winaux
     from vmbd_amd.olaGPU import OlaGPU
     sx, csf, sf, x, f, g = small_case
-    print(f'case OlaGPU deconv {mode}', flush=True)
+    print(f'case OlaGPU deconv {mode} fused={fused}', flush=True)
     wa = win2winaux(sx, csf, 0.5)
     owa = O.Win2WinAux(sx, csf, 0.5)
     x32, f32 = _f32(x), _f32(f)
@@ -286,9 +287,10 @@ def test_olagpu_closed_form_deconv(cuda, small_case, mode):
         return DO.sparse(Fo, _f32(y32), 0.0001)
 
     if mode == 'gdirect':
-        got = OlaGPU(x.astype(np.float32), sf, 'valid', wa).deconv(y32, mode='gdirect', alpha=0.01, beta=0.01)
+        got = OlaGPU(x.astype(np.float32), sf, 'valid', wa, fused=fused).deconv(y32, mode='gdirect', alpha=0.01,
+                                                                                beta=0.01)
     else:
-        F = OlaGPU(f.astype(np.float32), sx, 'valid', wa)
+        F = OlaGPU(f.astype(np.float32), sx, 'valid', wa, fused=fused)
         kw = dict(alpha=0.0001) if mode == 'sparse' else dict(alpha=0.05, beta=0.01)
         got = F.deconv(y32, mode=mode, **kw)
     ref = oracle()
@@ -300,7 +302,7 @@ def test_olagpu_closed_form_deconv(cuda, small_case, mode):
     finally:
         O.set_fft(*old)
     err = rel_l2(_n(got), ref)
-    print(f'{mode}: gpu {err:.2e}, fp32 oracle {d:.2e}', flush=True)
+    print(f'{mode} fused={fused}: gpu {err:.2e}, fp32 oracle {d:.2e}', flush=True)
     assert err <= max(10 * d, 1e-5)
 
 
@@ -355,3 +357,26 @@ def test_olagpu_objective_and_gradient_values(cuda, small_case, inst, alpha, bet
         # the Tikhonov term enters with the sign opposite to its gradient
         g0 = DO.cnvinv_gradfun(Zo, z, y32, alpha, 0.0, exact=False)
         assert rel_l2(grad.reshape(sz), g0 - beta * z) <= 1e-5
+
+
+@pytest.mark.parametrize('mode', ['valid', 'same'])
+def test_fused_xdirect_both_instances(cuda, mode):
+    """'xdirect' on the fused passes for an 'F' instance (image) and an 'X'
+    instance (PSFs), valid and same offsets, on a rectangular grid with
+    unequal PSF sides, against the oracle at 1e-5."""
+    from vmbd_amd.imagetools import win2winaux
+    from vmbd_amd.olaGPU import OlaGPU
+    sx, csf, sf = (120, 100), (3, 4), (7, 5)
+    print(f'case fused xdirect {mode}', flush=True)
+    x, f, _ = case_inputs(46, sx, csf, sf)
+    wa, owa = win2winaux(sx, csf, 0.5), O.Win2WinAux(sx, csf, 0.5)
+    x32, f32 = _f32(x), _f32(f)
+    y32 = _f32(O.forward(x32, f32, owa, mode) + 0.01 * np.random.default_rng(47).standard_normal(
+        O.out_size(sx, sf, mode)))
+    for alpha, beta in ((0.0, 0.02), (0.05, 0.01)):
+        F = OlaGPU(f.astype(np.float32), sx, mode, wa)
+        ref = DO.xdirect(O.OlaOracle(f32, sx, mode, owa), y32, alpha, beta)
+        assert rel_l2(_n(F.deconv(y32.astype(np.float32), mode='xdirect', alpha=alpha, beta=beta)), ref) <= 1e-5
+        X = OlaGPU(x.astype(np.float32), sf, mode, wa)
+        ref = DO.xdirect(O.OlaOracle(x32, sf, mode, owa), y32, alpha, beta)
+        assert rel_l2(_n(X.deconv(y32.astype(np.float32), mode='xdirect', alpha=alpha, beta=beta)), ref) <= 1e-5

@@ -349,3 +349,33 @@ def test_full_mode_transposes_dot_product():
     fb = O.adjoint_f(g, x, wa, sf, 'full', exact=True)
     assert abs(lhs - np.sum(x * xb)) < 1e-12 * abs(lhs)
     assert abs(lhs - np.sum(f * fb)) < 1e-12 * abs(lhs)
+
+
+def test_ola_flat_matches_kernel_loops():
+    """ola_flat against a literal transcription of the loops of
+    gputools.cu:295-314 (covering patch ci - d_i while i < wm + hop (ci - d_i),
+    flat read k*inm*inn + r*inn + c, reads past the stack give 0), with an
+    extent wm larger than the slices (the F.cnvtp call, olaGPU.py:307-309)."""
+    rng = np.random.default_rng(21)
+    csf, sw, hop = (2, 3), (6, 6), (3, 3)
+    xs = rng.random((6,) + sw)
+    wm = (8, 8)
+    flat = xs.ravel()
+    sx = O.sxp_of(csf, wm, hop)
+    ref = np.zeros((int(sx[0]), int(sx[1])))
+    for i in range(ref.shape[0]):
+        for j in range(ref.shape[1]):
+            ci = min(i // hop[0], csf[0] - 1)
+            cj = min(j // hop[1], csf[1] - 1)
+            d_i = 0
+            while i < wm[0] + hop[0] * (ci - d_i) and ci - d_i >= 0:
+                d_j = 0
+                while j < wm[1] + hop[1] * (cj - d_j) and cj - d_j >= 0:
+                    r = i + (d_i - ci) * hop[0]
+                    c = j + (d_j - cj) * hop[1]
+                    k = (ci - d_i) * csf[1] + (cj - d_j)
+                    e = k * sw[0] * sw[1] + r * sw[1] + c
+                    ref[i, j] += flat[e] if e < flat.size else 0.0
+                    d_j += 1
+                d_i += 1
+    np.testing.assert_allclose(O.ola_flat(xs, csf, wm, hop), ref, rtol=1e-14, atol=1e-14)

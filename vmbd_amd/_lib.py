@@ -55,6 +55,12 @@ SIGNATURES = {
     'vmbd_ola_adjoint_cached': (ctypes.c_int, [_P, _I, _I, _P, _F, _F, _I, _P]),
     'vmbd_ola_forward_cached_ex': (ctypes.c_int, [_P, _I, _I, _P, _F, _F, _F, _F, _P]),
     'vmbd_ola_adjoint_cached_ex': (ctypes.c_int, [_P, _I, _I, _P, _F, _F, ctypes.c_float, _F, _I, _P]),
+    'vmbd_deconv_workspace_bytes': (ctypes.c_size_t, [_P]),
+    'vmbd_deconv_direct': (ctypes.c_int, [_P, _I, _P, _F, ctypes.c_float, ctypes.c_float, _F, _P]),
+    'vmbd_deconv_gdirect': (ctypes.c_int, [_P, _F, _F, _F, _F, ctypes.c_float, ctypes.c_float, _F, _P, _P]),
+    'vmbd_deconv_sparse_setup': (ctypes.c_int, [_P, _P, _F, _F, ctypes.c_float, ctypes.c_float, _P, _F, _P, _P]),
+    'vmbd_deconv_sparse_step': (ctypes.c_int, [_P, _P, _P, _F, _I, _I, _F, _I, _I, ctypes.c_float, ctypes.c_float,
+                                               ctypes.c_float, _F, _F, _P, _P]),
     # plain (reference-decomposition) kernels, include/vmbd_plain.h
     'vmbd_pad': (ctypes.c_int, [_F, _I, _I, _F, _I, _I, _I, _I, _I, _P]),
     'vmbd_chop_pad': (ctypes.c_int, [_F, _I, _I, _F, _I, _I, _I, _I, _I, _I, _I, _I, _I, _I, _F, _F, _I, _P]),

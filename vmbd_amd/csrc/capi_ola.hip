@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../../include/vmbd.h"
+#include "../../include/vmbd_solver.h"
 #include "capi_util.h"
 #include "fft_core.h"
 #include "ola_kernels.h"
@@ -632,6 +633,191 @@ int vmbd_ola_adjoint_cached_ex(vmbd_plan* p, int32_t nframes, int32_t kind, cons
 int vmbd_ola_adjoint_cached(vmbd_plan* p, int32_t nframes, int32_t kind, const void* spec, const float* gres,
                             float* out, int32_t exact, vmbd_stream_t stream) {
   return vmbd_ola_adjoint_cached_ex(p, nframes, kind, spec, gres, nullptr, 0.f, out, exact, stream);
+}
+
+// ---------------------------------------------------------------------------
+// Closed-form deconvolutions (olaGPU.py:416-733): row pass, Fourier division
+// in the column pass, inverse rows into the output.  One frame.
+// ---------------------------------------------------------------------------
+namespace {
+
+// rows of one deconvolution operand in the workspace: slot j of four
+float2* work_rows(const vmbd_plan* p, void* work, int j) {
+  const size_t per = (size_t)p->g.Pf * p->g.NB * row_pitch(p->g.sw0);
+  return (float2*)work + (size_t)j * per;
+}
+
+// a patch of an (rows x cols) image placed at tile offset (off0, off1),
+// rows / columns [off, off + sw) cut to the tile (chop_pad, gputools.py:364-401)
+RowSrc src_chop(const vmbd_plan* p, const float* img, int rows, int cols, int off0, int off1) {
+  const Geo& g = p->g;
+  RowSrc s{};
+  s.src = img;
+  s.ld = cols;
+  s.srcH = rows;
+  s.srcW = cols;
+  s.origin = 1;
+  s.sh0 = off0;
+  s.sh1 = off1;
+  s.rlo = off0;
+  s.rhi = std::min(off0 + g.sw0, g.Th);
+  s.clo = off1;
+  s.chi = std::min(off1 + g.sw1, g.Tw);
+  if (s.rhi < s.rlo) s.rhi = s.rlo;
+  if (s.chi < s.clo) s.chi = s.clo;
+  return s;
+}
+
+// ... multiplied by the patch's window placed at tile offset (w0, w1); the
+// rows and columns outside the window are cut
+RowSrc windowed(const vmbd_plan* p, RowSrc s, int w0, int w1) {
+  const Geo& g = p->g;
+  s.win = p->win;
+  s.win_idx = p->win_idx;
+  s.wsh0 = w0;
+  s.wsh1 = w1;
+  s.rlo = std::max(s.rlo, w0);
+  s.rhi = std::min(s.rhi, w0 + g.sw0);
+  s.clo = std::max(s.clo, w1);
+  s.chi = std::min(s.chi, w1 + g.sw1);
+  if (s.rhi < s.rlo) s.rhi = s.rlo;
+  if (s.chi < s.clo) s.chi = s.clo;
+  return s;
+}
+
+// overlap-add of the patches' first sw rows and columns into an H x W image
+// (crop_stack to sw, optional window, ola, crop to sx)
+OlaDst dst_patches(const vmbd_plan* p, float* out, bool window) {
+  OlaDst d = dst_adjoint_x(p, out);
+  if (!window) {
+    d.win = nullptr;
+    d.win_idx = nullptr;
+    d.win_slot = nullptr;
+    d.row_class = nullptr;
+    d.nslot = 0;
+  }
+  return d;
+}
+
+int check_deconv(const vmbd_plan* p, const char* what) {
+  if (int r = check_plan(p, 1)) return r;
+  if (p->g.Pf <= 0) return vmbd_fail(VMBD_E_ARG, "%s: empty plan", what);
+  return VMBD_OK;
+}
+
+}  // namespace
+
+size_t vmbd_deconv_workspace_bytes(const vmbd_plan* p) {
+  if (!p) return 0;
+  return 4 * (size_t)p->g.Pf * p->g.NB * row_pitch(p->g.sw0) * sizeof(float2);
+}
+
+int vmbd_deconv_direct(vmbd_plan* p, int32_t kind, const void* spec, const float* y, float alpha, float beta,
+                       float* out, vmbd_stream_t stream) {
+  vmbd_clear_error();
+  if (int r = check_deconv(p, "vmbd_deconv_direct")) return r;
+  if (!spec || !y || !out) return vmbd_fail(VMBD_E_ARG, "vmbd_deconv_direct: null pointer");
+  if (kind != VMBD_CACHE_PSF && kind != VMBD_CACHE_IMAGE) return vmbd_fail(VMBD_E_ARG, "vmbd_deconv_direct: bad kind");
+  hipStream_t st = (hipStream_t)stream;
+  const Geo g = call_geo(p, 1);
+  p->rows_frames = 0;
+  // the observation's patch at the mode offset times the window at tile
+  // offset 0 (olaGPU.py:577-581)
+  const RowSrc s = windowed(p, src_chop(p, y, g.sy0, g.sy1, g.oc0, g.oc1), 0, 0);
+  VMBD_LAUNCH(launch_rows_r2c(g, s, p->wsX, p->tw, st));
+  const ColIn in[2] = {raw_in(p->wsX, s), spec_in(spec, g.Th)};
+  const DivParams prm{alpha, beta, 0.f};
+  if (kind == VMBD_CACHE_PSF) {
+    VMBD_LAUNCH(launch_cols_div(g, DIV_X, in, 2, prm, ColOut{p->Y1, g.sw0}, nullptr, p->tw, st));
+    VMBD_LAUNCH(launch_rows_c2r_ola(g, dst_patches(p, out, false), p->tw, st));
+  } else {
+    VMBD_LAUNCH(launch_cols_div(g, DIV_X, in, 2, prm, ColOut{p->Y2, g.sf0}, nullptr, p->tw, st));
+    VMBD_LAUNCH(launch_rows_c2r_stack(g, p->Y2, g.sf0, out, g.sf0, g.sf1, inv_scale(g), p->tw, st));
+  }
+  return VMBD_OK;
+}
+
+int vmbd_deconv_gdirect(vmbd_plan* p, const float* xx, const float* xy, const float* yx, const float* yy, float alpha,
+                        float beta, float* fout, void* work, vmbd_stream_t stream) {
+  vmbd_clear_error();
+  if (int r = check_deconv(p, "vmbd_deconv_gdirect")) return r;
+  if (!xx || !xy || !yx || !yy || !fout || !work) return vmbd_fail(VMBD_E_ARG, "vmbd_deconv_gdirect: null pointer");
+  hipStream_t st = (hipStream_t)stream;
+  const Geo g = call_geo(p, 1);
+  p->rows_frames = 0;
+  // gradients of the latent image at offset 0, of the observation at the
+  // mode offset (olaGPU.py:480-500)
+  const RowSrc srcs[4] = {src_chop(p, xx, g.H, g.W - 1, 0, 0), src_chop(p, yx, g.sy0, g.sy1 - 1, g.oc0, g.oc1),
+                          src_chop(p, xy, g.H - 1, g.W, 0, 0), src_chop(p, yy, g.sy0 - 1, g.sy1, g.oc0, g.oc1)};
+  float2* outs[4] = {work_rows(p, work, 0), work_rows(p, work, 1), work_rows(p, work, 2), work_rows(p, work, 3)};
+  VMBD_LAUNCH(launch_rows_r2c_multi(g, srcs, outs, 4, p->tw, st));
+  const ColIn in[4] = {raw_in(outs[0], srcs[0]), raw_in(outs[1], srcs[1]), raw_in(outs[2], srcs[2]),
+                       raw_in(outs[3], srcs[3])};
+  VMBD_LAUNCH(launch_cols_div(g, DIV_G, in, 4, DivParams{alpha, beta, 0.f}, ColOut{p->Y2, g.sf0}, nullptr, p->tw,
+                              st));
+  VMBD_LAUNCH(launch_rows_c2r_stack(g, p->Y2, g.sf0, fout, g.sf0, g.sf1, inv_scale(g), p->tw, st));
+  return VMBD_OK;
+}
+
+int vmbd_deconv_sparse_setup(vmbd_plan* p, const void* spec, const float* y, const float* c, float alpha, float gamma,
+                             void* ftpy, float* cfac, void* work, vmbd_stream_t stream) {
+  vmbd_clear_error();
+  if (int r = check_deconv(p, "vmbd_deconv_sparse_setup")) return r;
+  if (!spec || !y || !c || !ftpy || !cfac || !work)
+    return vmbd_fail(VMBD_E_ARG, "vmbd_deconv_sparse_setup: null pointer");
+  hipStream_t st = (hipStream_t)stream;
+  const Geo g = call_geo(p, 1);
+  p->rows_frames = 0;
+  // observation and constant image, patches and windows both at the mode
+  // offset (olaGPU.py:655-672)
+  const RowSrc srcs[2] = {windowed(p, src_chop(p, y, g.sy0, g.sy1, g.oc0, g.oc1), g.oc0, g.oc1),
+                          windowed(p, src_chop(p, c, g.sy0, g.sy1, g.oc0, g.oc1), g.oc0, g.oc1)};
+  float2* outs[2] = {work_rows(p, work, 0), work_rows(p, work, 1)};
+  VMBD_LAUNCH(launch_rows_r2c_multi(g, srcs, outs, 2, p->tw, st));
+  // F^H y, kept as a spectrum for every half-quadratic step (olaGPU.py:690-692)
+  const ColIn yin[2] = {raw_in(outs[0], srcs[0]), spec_in(spec, g.Th)};
+  VMBD_LAUNCH(launch_cols_div(g, DIV_SPROD, yin, 2, DivParams{0.f, 0.f, 0.f}, ColOut{nullptr, 0}, (float2*)ftpy,
+                              p->tw, st));
+  // deconvolved constant image (olaGPU.py:674-688): the filter energy of the
+  // sqrt(2)/2-scaled differences is L1/2, hence alpha/2 (exact)
+  const ColIn cin[2] = {raw_in(outs[1], srcs[1]), spec_in(spec, g.Th)};
+  VMBD_LAUNCH(launch_cols_div(g, DIV_X, cin, 2, DivParams{0.5f * alpha, gamma, 0.f}, ColOut{p->Y1, g.sw0}, nullptr,
+                              p->tw, st));
+  OlaDst d = dst_patches(p, cfac, true);
+  d.clip_lo = 1;
+  d.lo = 0.01f;
+  VMBD_LAUNCH(launch_rows_c2r_ola(g, d, p->tw, st));
+  // cfac = 0.01 / max(const, 0.01)
+  return vmbd_ew(cfac, cfac, nullptr, 0.01f, (int64_t)g.H * g.W, VMBD_EW_SDIV, stream);
+}
+
+int vmbd_deconv_sparse_step(vmbd_plan* p, const void* spec, const void* ftpy, const float* wx, int32_t wx_rows,
+                            int32_t wx_cols, const float* wy, int32_t wy_rows, int32_t wy_cols, float alpha,
+                            float beta, float gamma, const float* cfac, float* x, void* work, vmbd_stream_t stream) {
+  vmbd_clear_error();
+  if (int r = check_deconv(p, "vmbd_deconv_sparse_step")) return r;
+  if (!spec || !ftpy || !wx || !wy || !cfac || !x || !work)
+    return vmbd_fail(VMBD_E_ARG, "vmbd_deconv_sparse_step: null pointer");
+  if (wx_rows <= 0 || wx_cols <= 0 || wy_rows <= 0 || wy_cols <= 0 || !(alpha * beta > 0.f))
+    return vmbd_fail(VMBD_E_ARG, "vmbd_deconv_sparse_step: bad size or alpha*beta <= 0");
+  hipStream_t st = (hipStream_t)stream;
+  const Geo g = call_geo(p, 1);
+  p->rows_frames = 0;
+  // shrunk gradients at offset 0 (olaGPU.py:700-712)
+  const RowSrc srcs[2] = {src_chop(p, wx, wx_rows, wx_cols, 0, 0), src_chop(p, wy, wy_rows, wy_cols, 0, 0)};
+  float2* outs[2] = {work_rows(p, work, 0), work_rows(p, work, 1)};
+  VMBD_LAUNCH(launch_rows_r2c_multi(g, srcs, outs, 2, p->tw, st));
+  const ColIn in[4] = {raw_in(outs[0], srcs[0]), raw_in(outs[1], srcs[1]), spec_in(ftpy, g.Th), spec_in(spec, g.Th)};
+  VMBD_LAUNCH(launch_cols_div(g, DIV_S, in, 4, DivParams{alpha, beta, gamma}, ColOut{p->Y1, g.sw0}, nullptr, p->tw,
+                              st));
+  // windows, overlap-add, times the constant-image factor, clipped at 0
+  // (olaGPU.py:714-730)
+  OlaDst d = dst_patches(p, x, true);
+  d.mul = cfac;
+  d.clip_lo = 1;
+  d.lo = 0.f;
+  VMBD_LAUNCH(launch_rows_c2r_ola(g, d, p->tw, st));
+  return VMBD_OK;
 }
 
 }  // extern "C"

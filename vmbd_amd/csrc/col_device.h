@@ -1,0 +1,74 @@
+// Device helpers shared by the column-pass kernels (ola_kernels.hip,
+// deconv_kernels.hip): workgroup size, the column loads of row-pass output or
+// cached spectra, and the staged row-major store of a strip's columns.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "fft_core.h"
+#include "ola_kernels.h"
+
+namespace vmbd {
+
+constexpr int WG = 256;
+constexpr int NXCD = 8;  // XCDs of an MI355X; workgroups are dealt to them round-robin
+
+template <int A, int B>
+struct cmax {
+  static constexpr int v = A > B ? A : B;
+};
+
+// 16-byte global load of 4 consecutive floats at a 4-byte-aligned address
+// (gfx950 vector memory accepts unaligned addresses)
+__device__ __forceinline__ void ld4(float (&d)[4], const float* p) { __builtin_memcpy(d, p, 16); }
+
+// NZ < 16: only slots r < NZ can hold stored rows (rlo + nr <= NZ*TPT); the
+// rest are zero without a load
+template <int TH, int NZ = 16>
+__device__ __forceinline__ void col_load(float2 (&v)[16], const ColIn& in, int p, int c, int NB, int t, bool active,
+                                         int grp, int row0) {
+  constexpr int TPT = TH / 16;
+  if constexpr (NZ < 16) {
+#pragma unroll
+    for (int r = NZ; r < 16; ++r) v[r] = make_float2(0.f, 0.f);
+  }
+  if (in.spec) {
+    const float2* col = in.p + ((size_t)p * NB + c) * TH;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) v[r] = active ? col[t + TPT * r] : make_float2(0.f, 0.f);
+  } else {
+    const float2* col = in.shared ? in.p + ((size_t)grp * NB + c) * in.pitch + row0
+                                  : in.p + ((size_t)p * NB + c) * in.pitch;
+#pragma unroll
+    for (int r = 0; r < NZ; ++r) {
+      const int rr = t + TPT * r - in.rlo;
+      v[r] = (active && rr >= 0 && rr < in.nr) ? col[rr] : make_float2(0.f, 0.f);
+    }
+  }
+}
+
+// stage rows [0,nrows) of the strip in LDS and store row-major [P][nrows][NBP].
+// The staging area aliases the FFT slices: barriers on both sides, except
+// after the kernel's last store (LAST), where no slice is used again.
+// MM < 16: the caller guarantees nrows <= MM*TPT, so slots m >= MM are never
+// stored (and the transform's outputs there are dead code).
+template <int TH, int CS, bool LAST = false, int MM = 16>
+__device__ __forceinline__ void col_store_rows(const float2 (&v)[16], float2* st, const ColOut& o, int p, int strip,
+                                               int NB, int NBP, int cc, int t) {
+  constexpr int TPT = TH / 16;
+  __syncthreads();  // all transforms of the workgroup are done with the slices
+#pragma unroll
+  for (int m = 0; m < MM; ++m) {
+    const int r = t + TPT * m;
+    if (r < o.nrows) st[r * (CS + 1) + cc] = v[m];
+  }
+  __syncthreads();
+  const int c0 = strip * CS;
+  const int ci = threadIdx.x & (CS - 1);
+  if (c0 + ci < NB - 1) {  // bin NB-1 travels packed in bin 0
+    float2* dst = o.p + (size_t)p * o.nrows * NBP + c0 + ci;
+    for (int r = threadIdx.x / CS; r < o.nrows; r += WG / CS) dst[(size_t)r * NBP] = st[r * (CS + 1) + ci];
+  }
+  if (!LAST) __syncthreads();  // staging drained before the slices are reused
+}
+
+}  // namespace vmbd

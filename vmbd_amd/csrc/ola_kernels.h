@@ -30,7 +30,9 @@ struct Geo {
 // Tile element (r, c) of patch p reads src[p*pstride + (o0+r)*ld + (o1+c)]
 // with (o0, o1) = (origin ? (m*hop0, n*hop1) : (0,0)) - (sh0, sh1), zero when
 // (r, c) is outside [rlo,rhi) x [clo,chi) or the source index is outside
-// [0,srcH) x [0,srcW); multiplied by win[win_idx[p]][r][c] when win != null.
+// [0,srcH) x [0,srcW); multiplied by win[win_idx[p]][r - wsh0][c - wsh1]
+// when win != null (the window placed at tile offset (wsh0, wsh1); the
+// caller keeps [rlo,rhi) x [clo,chi) inside it).
 struct RowSrc {
   const float* src;
   long long pstride;  // elements between consecutive patches (PSF stacks)
@@ -41,14 +43,16 @@ struct RowSrc {
   int rlo, rhi, clo, chi;
   const float* win;
   const int* win_idx;
+  int wsh0, wsh1;
 };
 
-// Up to three row-transform jobs fused in one launch: blocks [0, blk_end[0])
+// Up to four row-transform jobs fused in one launch: blocks [0, blk_end[0])
 // run job 0, [blk_end[0], blk_end[1]) job 1, ...
+constexpr int ROW_JOBS_MAX = 4;
 struct RowJobs {
-  RowSrc s[3];
-  float2* out[3];
-  int blk_end[3];
+  RowSrc s[ROW_JOBS_MAX];
+  float2* out[ROW_JOBS_MAX];
+  int blk_end[ROW_JOBS_MAX];
   int n;
 };
 
@@ -107,18 +111,43 @@ struct OlaDst {
   const float* mul;
   const float* axz;
   float beta;
+  // then v = max(v, lo) when clip_lo != 0
+  int clip_lo;
+  float lo;
 };
 
 enum ColOp { COL_SPEC = 0, COL_FWD = 1, COL_ADJ1 = 2, COL_ADJ2 = 3 };
 
+// Fourier divisions of the closed-form deconvolutions (olaGPU.py:416-733,
+// gputools.cu:316-406), on the half spectra of the column pass, per
+// (row bin c, column frequency k) with wx = 2 pi c / Tw, wy = 2 pi k / Th:
+//   DIV_X  Z = conj(B) A / (|B|^2 + alpha L1 + beta)              (xdirect)
+//          L1 = 4 sin^2(wx/2) + 4 sin^2(wy/2): the [-1 1] and [-1 1]' filters
+//   DIV_G  Z = (conj(B1) A1 + conj(B2) A2) / (|B1|^2 + |B2|^2 + alpha L2 + beta)
+//          L2 = (1+2cos wy)^2 4 sin^2(wx/2) + 4 sin^2(wy/2) (1+2cos wx)^2  (gdirect)
+//   SPROD  S = A conj(B), written as a spectrum                   (sparse, F^H y)
+//   DIV_S  Z = (conj(Gx) A1 + conj(Gy) A2 + S1/(alpha beta)) / (|S2|^2/(alpha beta) + Ls + gamma)
+//          Gx = r(e^{-i wx} - 1), Gy = r(e^{-i wy} - 1), r = sqrt(2)/2, Ls = |Gx|^2 + |Gy|^2  (sparse)
+// A* are row-pass outputs (transformed here), B / S* cached spectra.
+enum DivOp { DIV_X = 0, DIV_G = 1, DIV_SPROD = 2, DIV_S = 3 };
+struct DivParams {
+  float alpha, beta, gamma;
+};
+
 // Launchers (ola_kernels.hip).  Return hipError_t of the launch.
 hipError_t launch_rows_r2c(const Geo& g, const RowSrc& s, float2* out, const float2* tw, hipStream_t st);
-// several row-transform jobs in one launch (n <= 3)
+// several row-transform jobs in one launch (n <= ROW_JOBS_MAX)
 hipError_t launch_rows_r2c_multi(const Geo& g, const RowSrc* s, float2* const* out, int n, const float2* tw,
                                  hipStream_t st);
 hipError_t launch_cols(const Geo& g, int op, const ColIn& a, const ColIn& b, const ColIn& c, const ColOut& o1,
                        const ColOut& o2, float2* spec_out, const float2* tw, hipStream_t st);
 hipError_t launch_rows_c2r_ola(const Geo& g, const OlaDst& d, const float2* tw, hipStream_t st);
+// Column pass of a Fourier division (deconv_kernels.hip): operands a[0..na)
+// (row-pass outputs or spectra, in the order of DivOp's formula: DIV_X /
+// SPROD: A, B; DIV_G: B1, A1, B2, A2; DIV_S: A1, A2, S1, S2); result rows
+// [0, o.nrows) to o, or (SPROD) the spectrum to spec_out.
+hipError_t launch_cols_div(const Geo& g, int op, const ColIn* a, int na, DivParams prm, const ColOut& o,
+                           float2* spec_out, const float2* tw, hipStream_t st);
 // out = scale * inverse row transforms (+ beta * axz when axz != null)
 hipError_t launch_rows_c2r_stack(const Geo& g, const float2* src, int srow, float* out, int nrows, int ncols,
                                  float scale, const float2* tw, hipStream_t st, const float* axz = nullptr, float beta = 0.f);

@@ -26,23 +26,14 @@
 // LDS: each workgroup owns one buffer; the FFT slices and the staging area of
 // the transposed store alias it (separated by workgroup barriers), which keeps
 // the footprint at 35-40 KB and allows four workgroups per CU.
+#include "col_device.h"
 #include "fft_core.h"
 #include "ola_kernels.h"
 #include "prof.h"
 
 namespace vmbd {
 
-constexpr int WG = 256;
-constexpr int NXCD = 8;  // XCDs of an MI355X; workgroups are dealt to them round-robin
 
-template <int A, int B>
-struct cmax {
-  static constexpr int v = A > B ? A : B;
-};
-
-// 16-byte global load of 4 consecutive floats at a 4-byte-aligned address
-// (gfx950 vector memory accepts unaligned addresses)
-__device__ __forceinline__ void ld4(float (&d)[4], const float* p) { __builtin_memcpy(d, p, 16); }
 
 // ----------------------------------------------------------------------------
 // rows_r2c
@@ -74,14 +65,12 @@ __global__ __launch_bounds__(WG) void k_rows_r2c(Geo g, RowJobs jobs, const floa
   const int pu = tile * TP + (ur - by * Tt);
   int ji = 0;
   int blk0 = 0;
-  if (jobs.n > 1 && by >= jobs.blk_end[0]) {
-    ji = 1;
-    blk0 = jobs.blk_end[0];
-    if (jobs.n > 2 && by >= jobs.blk_end[1]) {
-      ji = 2;
-      blk0 = jobs.blk_end[1];
+#pragma unroll
+  for (int jj = 0; jj < ROW_JOBS_MAX - 1; ++jj)
+    if (jobs.n > jj + 1 && by >= jobs.blk_end[jj]) {
+      ji = jj + 1;
+      blk0 = jobs.blk_end[jj];
     }
-  }
   const RowSrc s = jobs.s[ji];
   float2* __restrict__ out = jobs.out[ji];
   const int q0 = (by - blk0) * TPW;  // first row pair of the block
@@ -115,7 +104,8 @@ __global__ __launch_bounds__(WG) void k_rows_r2c(Geo g, RowJobs jobs, const floa
   const bool ok1 = active && r0 + 1 < s.rhi && i1 >= 0 && i1 < s.srcH;
   const float* row0 = base + (size_t)(ok0 ? i0 : 0) * s.ld;
   const float* row1 = base + (size_t)(ok1 ? i1 : 0) * s.ld;
-  const float* w0 = wb ? wb + (size_t)r0 * g.sw1 : nullptr;
+  // window row r0 - wsh0, column c - wsh1 (only read inside [clo,chi))
+  const float* w0 = wb ? wb + ((long long)(r0 - s.wsh0) * g.sw1 - s.wsh1) : nullptr;
 
   float2* sl = sm + tr * SL;
   float2 za[9], zb[9];
@@ -243,7 +233,7 @@ static hipError_t rows_r2c_t(const Geo& g, const RowSrc* s, float2* const* out, 
 
 hipError_t launch_rows_r2c_multi(const Geo& g, const RowSrc* s, float2* const* out, int n, const float2* tw,
                                  hipStream_t st) {
-  if (n < 1 || n > 3) return hipErrorInvalidValue;
+  if (n < 1 || n > ROW_JOBS_MAX) return hipErrorInvalidValue;
   switch (g.Tw) {
     case 16: return rows_r2c_t<16>(g, s, out, n, tw, st);
     case 32: return rows_r2c_t<32>(g, s, out, n, tw, st);
@@ -262,56 +252,6 @@ hipError_t launch_rows_r2c(const Geo& g, const RowSrc& s, float2* out, const flo
 // ----------------------------------------------------------------------------
 // cols
 // ----------------------------------------------------------------------------
-// NZ < 16: only slots r < NZ can hold stored rows (rlo + nr <= NZ*TPT); the
-// rest are zero without a load
-template <int TH, int NZ = 16>
-__device__ __forceinline__ void col_load(float2 (&v)[16], const ColIn& in, int p, int c, int NB, int t, bool active,
-                                         int grp, int row0) {
-  constexpr int TPT = TH / 16;
-  if constexpr (NZ < 16) {
-#pragma unroll
-    for (int r = NZ; r < 16; ++r) v[r] = make_float2(0.f, 0.f);
-  }
-  if (in.spec) {
-    const float2* col = in.p + ((size_t)p * NB + c) * TH;
-#pragma unroll
-    for (int r = 0; r < 16; ++r) v[r] = active ? col[t + TPT * r] : make_float2(0.f, 0.f);
-  } else {
-    const float2* col = in.shared ? in.p + ((size_t)grp * NB + c) * in.pitch + row0
-                                  : in.p + ((size_t)p * NB + c) * in.pitch;
-#pragma unroll
-    for (int r = 0; r < NZ; ++r) {
-      const int rr = t + TPT * r - in.rlo;
-      v[r] = (active && rr >= 0 && rr < in.nr) ? col[rr] : make_float2(0.f, 0.f);
-    }
-  }
-}
-
-// stage rows [0,nrows) of the strip in LDS and store row-major [P][nrows][NBP].
-// The staging area aliases the FFT slices: barriers on both sides, except
-// after the kernel's last store (LAST), where no slice is used again.
-// MM < 16: the caller guarantees nrows <= MM*TPT, so slots m >= MM are never
-// stored (and the transform's outputs there are dead code).
-template <int TH, int CS, bool LAST = false, int MM = 16>
-__device__ __forceinline__ void col_store_rows(const float2 (&v)[16], float2* st, const ColOut& o, int p, int strip,
-                                               int NB, int NBP, int cc, int t) {
-  constexpr int TPT = TH / 16;
-  __syncthreads();  // all transforms of the workgroup are done with the slices
-#pragma unroll
-  for (int m = 0; m < MM; ++m) {
-    const int r = t + TPT * m;
-    if (r < o.nrows) st[r * (CS + 1) + cc] = v[m];
-  }
-  __syncthreads();
-  const int c0 = strip * CS;
-  const int ci = threadIdx.x & (CS - 1);
-  if (c0 + ci < NB - 1) {  // bin NB-1 travels packed in bin 0
-    float2* dst = o.p + (size_t)p * o.nrows * NBP + c0 + ci;
-    for (int r = threadIdx.x / CS; r < o.nrows; r += WG / CS) dst[(size_t)r * NBP] = st[r * (CS + 1) + ci];
-  }
-  if (!LAST) __syncthreads();  // staging drained before the slices are reused
-}
-
 // Spectrum product of one column: B = A * B (forward) or A * conj(B)
 // (adjoints); the inverse transforms' 1/(Th*Tw) is applied to the summed
 // output by the final stores (a power of two: exact).  The column of bin 0 (packed; the first TPT
@@ -810,6 +750,7 @@ __global__ __launch_bounds__(WGS, (TW >= 512 ? 3 : 1)) void k_rows_c2r_ola(Geo g
       if (d.sub) a -= d.sub[o];
       if (d.mul) a *= d.mul[o];
       if (d.axz) a += d.beta * d.axz[o];
+      if (d.clip_lo) a = fmaxf(a, d.lo);
       d.out[o] = a;
     }
     return;
@@ -838,6 +779,7 @@ __global__ __launch_bounds__(WGS, (TW >= 512 ? 3 : 1)) void k_rows_c2r_ola(Geo g
     if (d.sub) acc -= d.sub[o];
     if (d.mul) acc *= d.mul[o];
     if (d.axz) acc += d.beta * d.axz[o];
+    if (d.clip_lo) acc = fmaxf(acc, d.lo);
     d.out[o] = acc;
   }
 }

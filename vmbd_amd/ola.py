@@ -204,6 +204,88 @@ class OlaPlan:
         self._rows_key = self._key(x, f, nf)
         return xb, fb
 
+    # -- closed-form deconvolutions (OlaGPU.deconv) --------------------------
+    def _deconv_work(self):
+        if getattr(self, '_work', None) is None:
+            n = int(_lib.lib().vmbd_deconv_workspace_bytes(self._h))
+            self._work = torch.empty(max(n, 16), dtype=torch.uint8, device=self.device)
+        return self._work
+
+    def _spec1(self, spec):
+        if self._spec_frames(spec) != 1:
+            raise ValueError('the deconvolutions take one frame')
+        return spec
+
+    def deconv_direct(self, kind, spec, y, alpha, beta, out=None):
+        """'xdirect' (olaGPU.py:550-609) with the cached spectrum of an 'F'
+        (kind 'F': returns the image) or 'X' instance (kind 'X': the PSFs)."""
+        y, nf, bat = self._g(y)
+        if bat:
+            raise ValueError('the deconvolutions take one frame')
+        self._spec1(spec)
+        k = _lib.CACHE_PSF if kind == 'F' else _lib.CACHE_IMAGE
+        shape = self.sx if kind == 'F' else (self.P,) + self.sf
+        o = self._empty(shape, out)
+        self._rows_key = None
+        _lib.check(_lib.lib().vmbd_deconv_direct(self._h, k, _lib.ptr(spec), _lib.ptr(y), float(alpha), float(beta),
+                                                 _lib.ptr(o), _lib.stream_ptr()), 'deconv_direct')
+        return o
+
+    def deconv_gdirect(self, xx, xy, yx, yy, alpha, beta, out=None):
+        """'gdirect' (olaGPU.py:416-511) from the valid-mode gradients of the
+        latent image (xx, xy) and of the tapered observation (yx, yy)."""
+        H, W = self.sx
+        s0, s1 = self.sy
+        want = dict(xx=(H, W - 1), xy=(H - 1, W), yx=(s0, s1 - 1), yy=(s0 - 1, s1))
+        ts = {}
+        for name, t in (('xx', xx), ('xy', xy), ('yx', yx), ('yy', yy)):
+            t = _as_dev(t, self.device, name)
+            if tuple(t.shape) != want[name]:
+                raise ValueError(f'{name} must have shape {want[name]}')
+            ts[name] = t
+        o = self._empty((self.P,) + self.sf, out)
+        self._rows_key = None
+        _lib.check(_lib.lib().vmbd_deconv_gdirect(self._h, _lib.ptr(ts['xx']), _lib.ptr(ts['xy']), _lib.ptr(ts['yx']),
+                                                  _lib.ptr(ts['yy']), float(alpha), float(beta), _lib.ptr(o),
+                                                  _lib.ptr(self._deconv_work()), _lib.stream_ptr()), 'deconv_gdirect')
+        return o
+
+    def sparse_setup(self, spec, y, c, alpha, gamma):
+        """'sparse' set-up (olaGPU.py:655-692): (F^H y spectrum, constant-image
+        factor 0.01 / max(deconvolved c, 0.01)) from the tapered observation y
+        and the tapered constant image c."""
+        self._spec1(spec)
+        y, _, by = self._g(y)
+        c, _, bc = self._g(c)
+        if by or bc:
+            raise ValueError('the deconvolutions take one frame')
+        ftpy = torch.empty_like(spec)
+        cfac = torch.empty(self.sx, dtype=torch.float32, device=self.device)
+        self._rows_key = None
+        _lib.check(_lib.lib().vmbd_deconv_sparse_setup(self._h, _lib.ptr(spec), _lib.ptr(y), _lib.ptr(c),
+                                                       float(alpha), float(gamma), _lib.ptr(ftpy), _lib.ptr(cfac),
+                                                       _lib.ptr(self._deconv_work()), _lib.stream_ptr()),
+                   'sparse_setup')
+        return ftpy, cfac
+
+    def sparse_step(self, spec, ftpy, wx, wy, alpha, beta, gamma, cfac, out=None):
+        """One half-quadratic x-step (olaGPU.py:700-730) from the shrunk
+        gradients wx, wy of the current estimate."""
+        self._spec1(spec)
+        self._spec1(ftpy)
+        wx = _as_dev(wx, self.device, 'wx')
+        wy = _as_dev(wy, self.device, 'wy')
+        if wx.dim() != 2 or wy.dim() != 2:
+            raise ValueError('wx and wy must be 2-D')
+        cfac = self._like(cfac, self.sx, 'cfac')
+        o = self._empty(self.sx, out)
+        self._rows_key = None
+        _lib.check(_lib.lib().vmbd_deconv_sparse_step(
+            self._h, _lib.ptr(spec), _lib.ptr(ftpy), _lib.ptr(wx), wx.shape[0], wx.shape[1], _lib.ptr(wy),
+            wy.shape[0], wy.shape[1], float(alpha), float(beta), float(gamma), _lib.ptr(cfac), _lib.ptr(o),
+            _lib.ptr(self._deconv_work()), _lib.stream_ptr()), 'sparse_step')
+        return o
+
     def windows(self):
         """The (P, sw0, sw1) normalised windows as the kernels apply them."""
         ws = torch.empty((self.P,) + self.sw, dtype=torch.float32, device=self.device)

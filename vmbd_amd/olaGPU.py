@@ -12,9 +12,13 @@ Differences from the reference, all deliberate:
     output, as in the reference (olaGPU.py:254-257);
   * the cached spectrum (``fft_gpu``) is the half spectrum of the real
     patches in the kernels' layout (P, Tw/2+1, Th) complex64 (the real row
-    bins 0 and Tw/2 packed into bin 0, bin Tw/2 zero); the
-    closed-form deconvolutions build the reference's full (P, Th, Tw)
-    spectrum on first use (``full_spectrum``);
+    bins 0 and Tw/2 packed into bin 0, bin Tw/2 zero);
+  * the closed-form deconvolutions ('xdirect', 'direct', 'gdirect',
+    'sparse') form their Fourier divisions inside the operator's column pass
+    on those half spectra (three launches per division); ``fused=False``
+    runs the reference's sequence instead (chop, pad, full (P, Th, Tw)
+    spectra built on first use by ``full_spectrum``, FFT, divide, IFFT, crop,
+    OLA), one plain kernel per step;
   * ``cnvtp`` of an 'F' instance overlap-adds with the patch extent sw (the
     reference passes sw+sf-1 and reads past every patch, olaGPU.py:307-309);
     ``exact=True`` gives the true adjoint instead of the patch-local transpose;
@@ -40,12 +44,15 @@ def _spectral_energy(gx, gy):
 
 
 class OlaGPU:
-    def __init__(self, f, sx, mode, winaux, exact=False, max_frames=1):
+    def __init__(self, f, sx, mode, winaux, exact=False, max_frames=1, fused=True):
         sf = np.array(f.shape)[-2:]
         sx = np.array(sx)
         sw = np.asarray(winaux.sw)
         csf = winaux.csf
         self.exact = bool(exact)
+        # closed-form deconvolutions: fused into the operator's passes, or the
+        # reference's sequence of plain kernels over full complex spectra
+        self.fused = bool(fused)
         if len(f.shape) == 3 and np.all(sx > sf):                     # olaGPU.py:104-111
             self.f = f
             self.x = []
@@ -233,6 +240,13 @@ class OlaGPU:
         """olaGPU.py:416-511: PSFs from gradient images by one Fourier division."""
         if self.__id__ != 'X':
             raise Exception('Use direct mode for image estimation!')
+        self._offset()
+        if self.fused:
+            if edgetapering == 1:
+                gputools.edgetaper_gpu(y_gpu, 2 * self.sf, 'barthann')
+            xx, xy = gputools.gradient_gpu(gputools._dev(self.x))
+            yx, yy = gputools.gradient_gpu(y_gpu)
+            return self.plan.deconv_gdirect(xx, xy, yx, yy, alpha, beta)
         full = self.full_spectrum()
         if alpha > 0.:
             gx = self._filter_spectrum([[-1, 1], [-1, 1], [-1, 1]])
@@ -279,6 +293,10 @@ class OlaGPU:
         reference, the tapered copy is not the one that gets chopped
         (olaGPU.py:566 vs 577) and the windows sit at tile offset 0 while the
         residual patches sit at the mode offset (olaGPU.py:577-581)."""
+        off = self._offset()
+        if self.fused:
+            # the tapered copy is never chopped: nothing to taper
+            return self.plan.deconv_direct(self.__id__, self.fft_gpu, gputools._dev(y), alpha, beta)
         full = self.full_spectrum()
         if alpha > 0.:
             gx = self._filter_spectrum([[-1, 1]])
@@ -288,7 +306,6 @@ class OlaGPU:
             L = torch.zeros(full.shape[-2:], dtype=torch.complex64, device=full.device)
         if edgetapering == 1:
             gputools.edgetaper_gpu(y_gpu, 3 * self.sf, 'barthann')
-        off = self._offset()
         wa = self.winaux
         yp = gputools.chop_pad_GPU(y, wa.csf, wa.sw, wa.nhop, self.sfft_gpu, off, 'complex')
         ws = gputools.pad_stack_GPU(wa.ws_gpu, self.sfft_gpu, dtype='complex')
@@ -306,6 +323,8 @@ class OlaGPU:
     def _deconv_sparse(self, y_gpu, alpha, gamma, edgetapering):
         """olaGPU.py:613-733: hyper-Laplacian (|grad x|^(2/3)) deconvolution
         by half-quadratic splitting (Krishnan & Fergus, NIPS 2009)."""
+        if self.fused:
+            return self._deconv_sparse_fused(y_gpu, alpha, gamma, edgetapering)
         full = self.full_spectrum()
         r = np.sqrt(2.) / 2.
         gx = self._filter_spectrum(r * np.array([[-1, 1]]))
@@ -350,6 +369,29 @@ class OlaGPU:
             x_gpu = gputools.crop_gpu2cpu(gputools.ola_GPU_test(zs, wa.csf, wa.sw, wa.nhop), self.sx)
             x_gpu = gputools.ew(gputools.EW_MUL, x_gpu, const)
             gputools.cliplower_GPU(x_gpu, 0.)
+            beta *= beta_rate
+        return x_gpu
+
+    def _deconv_sparse_fused(self, y_gpu, alpha, gamma, edgetapering):
+        """The same splitting on the fused passes: one set-up (F^H y and the
+        constant-image factor) and, per beta, the shrunk gradients of the
+        estimate through one row pass, one division and one overlap-add."""
+        if self.__id__ != 'F':
+            raise NotImplementedError('sparse deconvolution estimates the image (F instance)')
+        self._offset()
+        const = torch.ones_like(y_gpu)
+        if edgetapering == 1:
+            gputools.edgetaper_gpu(y_gpu, 2 * self.sf, 'barthann')
+            gputools.edgetaper_gpu(const, 2 * self.sf, 'barthann')
+        ftpy, cfac = self.plan.sparse_setup(self.fft_gpu, y_gpu, const, alpha, gamma)
+        beta, beta_rate, beta_max = 1., 2. * np.sqrt(2.), 2. ** 8
+        x_gpu = y_gpu.clone()
+        out = torch.empty(tuple(int(v) for v in self.sx), dtype=torch.float32, device=y_gpu.device)
+        while beta < beta_max:
+            xx, xy = gputools.gradient_gpu(x_gpu)
+            gputools.modify_sparse23_gpu(xx, beta)
+            gputools.modify_sparse23_gpu(xy, beta)
+            x_gpu = self.plan.sparse_step(self.fft_gpu, ftpy, xx, xy, alpha, beta, gamma, cfac, out=out)
             beta *= beta_rate
         return x_gpu
 
