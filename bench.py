@@ -17,7 +17,7 @@ and the online MFBD loop in frames/s (1024^2 latent, 20 projected-BB
 f-iterations + 20 x-iterations per frame, graph-captured).
 
     python bench.py [--steps K] [--warmup W] [--frames B] [--no-graph] [--cpu-seconds S]
-                    [--no-plain] [--no-mfbd] [--mfbd-frames N] [--no-mixed]
+                    [--no-plain] [--no-mfbd] [--mfbd-frames N] [--no-mixed] [--no-deconv]
 """
 import argparse
 import json
@@ -49,6 +49,7 @@ def parse():
     ap.add_argument('--no-mfbd', action='store_true', help='skip the MFBD frames/s measurement')
     ap.add_argument('--mfbd-frames', type=int, default=10, help='timed MFBD frames')
     ap.add_argument('--no-mixed', action='store_true', help='skip the 64-frame mixed-geometry batch')
+    ap.add_argument('--no-deconv', action='store_true', help='skip the closed-form deconvolution timings')
     ap.add_argument('--calib', action='store_true',
                     help='run one 64 MiB elementwise copy first (counter calibration)')
     return ap.parse_args()
@@ -242,10 +243,40 @@ C4_GEOMS = [((512, 512), (12, 12), (15, 15)), ((512, 512), (6, 6), (41, 41)), ((
             ((768, 1024), (6, 6), (15, 15)), ((1024, 1024), (10, 10), (41, 41)), ((1024, 1024), (12, 12), (25, 25))]
 
 
+def deconv_rates(wa, x, f, timed, reps=5):
+    """ms per OlaGPU.deconv call at the benchmark geometry (1024^2, 10 x 10
+    patches, 41 x 41 PSFs) for the closed-form modes, fused into the
+    operator's passes and as the reference's plain launch sequence, on an
+    observation y = B(x, f) + N(0, 0.01^2).  Each call includes its host-side
+    set-up (tapering, gradients, output allocation)."""
+    import numpy as np
+    import torch
+
+    from vmbd_amd.olaGPU import OlaGPU
+
+    y = OlaGPU(f, (H, W), 'valid', wa).cnv(x)
+    y = y + 0.01 * torch.randn(y.shape, generator=torch.Generator(device=y.device).manual_seed(5), device=y.device)
+    out = {}
+    for mode, inst, kw in (('xdirect', 'F', dict(alpha=0.05, beta=0.01)), ('direct', 'F', dict(alpha=0.05, beta=0.01)),
+                           ('gdirect', 'X', dict(alpha=0.01, beta=0.01)), ('sparse', 'F', dict(alpha=1e-4))):
+        e = {}
+        for fused in (True, False):
+            Z = OlaGPU(f, (H, W), 'valid', wa, fused=fused) if inst == 'F' else OlaGPU(x, SF, 'valid', wa, fused=fused)
+            Z.deconv(y, mode=mode, **kw)
+            e['fused_ms' if fused else 'plain_ms'] = round(timed(lambda: Z.deconv(y, mode=mode, **kw), reps), 3)
+            del Z
+        e['speedup'] = round(e['plain_ms'] / e['fused_ms'], 2)
+        out[mode] = e
+    return out
+
+
 def mixed_rate(nframes, timed, steps):
     """64 frames of six geometries (512^2 / 768x1024 / 1024^2, 6x6..12x12
     patches, 15/25/41 PSFs): forward + both exact adjoints of every frame,
-    one graph replay per step."""
+    one graph replay per step, in three launch forms: each pass of all groups
+    as one launch per distinct tile (grouped, one stream), one such batch per
+    tile forked over three streams, and each group's own launches forked
+    over three streams."""
     import numpy as np
     import torch
 
@@ -255,21 +286,32 @@ def mixed_rate(nframes, timed, steps):
     dev = torch.device('cuda', 0)
     rng = np.random.default_rng(7)
     specs = [C4_GEOMS[i % len(C4_GEOMS)] for i in range(nframes)]
-    mb = MixedBatch(specs, device=dev)
     to = lambda v: torch.as_tensor(np.ascontiguousarray(v, dtype=np.float32), device=dev)
     xs, fs, gs = [], [], []
     for sx, csf, sf in specs:
         xs.append(to(synth.piecewise_smooth(sx, rng)))
         fs.append(to(synth.motion_psfs(csf[0] * csf[1], sf, rng, steps=sf[0])))
         gs.append(to(0.01 * rng.standard_normal((sx[0] - sf[0] + 1, sx[1] - sf[1] + 1))))
-    mb.load(xs, fs, gs)
-    replay = mb.capture()
-    for _ in range(2):
-        replay()
-    ms = timed(replay, steps)
-    return {'frames': nframes, 'geometries': [[list(a), list(b), list(c)] for a, b, c in C4_GEOMS],
-            'groups': len(mb.groups), 'streams': mb.streams, 'ms_per_step': round(ms, 3), 'frames_per_s': round(nframes / (ms * 1e-3), 1),
-            'mpix_s': round(mb.pixels / (ms * 1e-3) / 1e6, 1), 'graph': True}
+    out = {'frames': nframes, 'geometries': [[list(a), list(b), list(c)] for a, b, c in C4_GEOMS], 'graph': True}
+    outs = {}
+    for name, kw in (('grouped', dict(grouped=True, streams=1)), ('grouped_streams3', dict(grouped=True, streams=3)),
+                     ('streams3', dict(grouped=False, streams=3))):
+        mb = MixedBatch(specs, device=dev, **kw)
+        mb.load(xs, fs, gs)
+        replay = mb.capture()
+        for _ in range(2):
+            replay()
+        ms = timed(replay, steps)
+        out[name] = {'ms_per_step': round(ms, 3), 'frames_per_s': round(nframes / (ms * 1e-3), 1),
+                     'mpix_s': round(mb.pixels / (ms * 1e-3) / 1e6, 1), 'launches_per_step': mb.launches,
+                     'groups': len(mb.groups)}
+        outs[name] = [tuple(t.clone() for t in mb.outputs(i)) for i in range(nframes)]
+        del mb
+    out['same_bits'] = all(torch.equal(a, b) for k in ('grouped', 'grouped_streams3') for i in range(nframes)
+                           for a, b in zip(outs[k][i], outs['streams3'][i]))
+    best = min(('grouped', 'grouped_streams3', 'streams3'), key=lambda k: out[k]['ms_per_step'])
+    out.update(best=best, mpix_s=out[best]['mpix_s'], frames_per_s=out[best]['frames_per_s'])
+    return out
 
 
 def main():
@@ -388,6 +430,10 @@ def main():
     if not a.no_mixed:
         mixed = mixed_rate(64, timed, max(3, min(a.steps, 10)))
 
+    deconv = None
+    if not a.no_deconv:
+        deconv = deconv_rates(wa, x[0], f[0], timed)
+
     mfbd = None
     if not a.no_mfbd:
         mfbd = {'1024': mfbd_rate((H, W), CSF, SF, 40, a.mfbd_frames, timed),
@@ -475,6 +521,7 @@ def main():
         'mfbd_frames_per_s': mfbd['1024']['frames_per_s'] if mfbd else None,
         'mfbd': mfbd,
         'mixed_batch': mixed,
+        'deconv': deconv,
     }
     print(json.dumps(out))
 

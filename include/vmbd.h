@@ -206,6 +206,28 @@ int vmbd_deconv_sparse_step(vmbd_plan* plan, const void* spec, const void* ftpy,
                             int32_t wx_cols, const float* wy, int32_t wy_rows, int32_t wy_cols, float alpha,
                             float beta, float gamma, const float* cfac, float* x, void* work, vmbd_stream_t stream);
 
+/* Batches of frames of different geometry (no counterpart in the reference,
+ * which runs one OlaGPU instance at a time; its batched pyfft call over all
+ * patches of one instance is olaGPU.py:813-824).  A batch binds n plans
+ * (each with its own geometry, tile and PSF size) and, per plan, nframes[i]
+ * resident frames: x[i], f[i], g[i] in and y[i], xbar[i], fbar[i] out, laid
+ * out as for the batched operators of that plan.  vmbd_batch_step runs the
+ * forward and both transposes (exact or patch-local) of every frame of every
+ * plan, each pass of all plans as one launch over a device table of groups
+ * (one launch per pass and distinct tile: the row, overlap-add and stack
+ * passes are specialised to Tw, the column passes to Th), on `stream`.  The
+ * plans and buffers must outlive the batch; the batch uses the plans'
+ * workspaces, so calls on those plans and steps of the batch must be
+ * ordered.  At most 16 plans; all on the current device. */
+typedef struct vmbd_batch vmbd_batch;
+vmbd_batch* vmbd_batch_create(vmbd_plan* const* plans, int32_t n, const int32_t* nframes, const float* const* x,
+                              const float* const* f, const float* const* g, float* const* y, float* const* xbar,
+                              float* const* fbar, int32_t exact);
+void vmbd_batch_destroy(vmbd_batch* batch);
+/* launches per step (passes x distinct tiles) */
+int vmbd_batch_launches(const vmbd_batch* batch);
+int vmbd_batch_step(vmbd_batch* batch, vmbd_stream_t stream);
+
 /* Opt-in per-kernel timing (no counterpart in the reference, whose only
  * timing is wall-clock prints, stopwatch.py:5-22).  When enabled, every kernel
  * launch is bracketed by HIP events on its stream; vmbd_prof_report()

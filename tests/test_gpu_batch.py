@@ -25,15 +25,26 @@ def _frames(cuda):
     return xs, fs, gs
 
 
-def test_mixed_batch_vs_oracle_and_single_plans(cuda):
+@pytest.mark.parametrize('grouped', [True, False], ids=['grouped', 'per-group'])
+def test_mixed_batch_vs_oracle_and_single_plans(cuda, grouped):
     import torch
     from vmbd_amd.batch import MixedBatch
     from vmbd_amd.imagetools import win2winaux
     from vmbd_amd.ola import OlaPlan
-    print('case mixed batch', flush=True)
+    print(f'case mixed batch grouped={grouped}', flush=True)
     xs, fs, gs = _frames(cuda)
-    mb = MixedBatch(SPECS)
+    mb = MixedBatch(SPECS, grouped=grouped)
     assert len(mb.groups) == 3
+    # grouped over three streams: one batch per tile, seven launches each
+    assert mb.launches == (7 * len({G['plan'].tile for G in mb.groups}) if grouped else 21)
+    if grouped:
+        # on one stream one batch of all groups: one launch per pass and
+        # distinct tile (width for the five row, overlap-add and stack
+        # passes, height for the two column passes)
+        m1 = MixedBatch(SPECS, grouped=True, streams=1)
+        n_tw = len({G['plan'].tile[1] for G in m1.groups})
+        n_th = len({G['plan'].tile[0] for G in m1.groups})
+        assert m1.launches == 5 * n_tw + 2 * n_th
     mb.load(xs, fs, gs)
     mb.step()
     torch.cuda.synchronize()
@@ -51,15 +62,15 @@ def test_mixed_batch_vs_oracle_and_single_plans(cuda):
         assert torch.equal(xb1, xb) and torch.equal(fb1, fb), i
 
 
-@pytest.mark.parametrize('streams', [1, 2, 3])
-def test_mixed_batch_graph_replay(cuda, streams):
+@pytest.mark.parametrize('streams,grouped', [(1, False), (2, False), (3, False), (1, True), (3, True)])
+def test_mixed_batch_graph_replay(cuda, streams, grouped):
     """The captured step equals the eager one, also with the groups forked
-    over several streams inside the graph."""
+    over several streams inside the graph, and with the grouped launches."""
     import torch
     from vmbd_amd.batch import MixedBatch
-    print(f'case mixed batch graph, {streams} streams', flush=True)
+    print(f'case mixed batch graph, {streams} streams, grouped={grouped}', flush=True)
     xs, fs, gs = _frames(cuda)
-    mb = MixedBatch(SPECS, streams=streams)
+    mb = MixedBatch(SPECS, streams=streams, grouped=grouped)
     mb.load(xs, fs, gs)
     mb.step()
     torch.cuda.synchronize()

@@ -92,19 +92,20 @@ def test_mixed_batch_stream_lanes():
     import torch
     from vmbd_amd.batch import MixedBatch
 
-    def fake(streams, groups):
+    def lanes(streams, groups):
         mb = MixedBatch.__new__(MixedBatch)
         mb.streams = streams
         mb.groups = [dict(idx=list(range(n)), x=torch.zeros((n,) + sx)) for n, sx in groups]
-        return mb
+        work = lambda gi: len(mb.groups[gi]['idx']) * mb.groups[gi]['x'][0].numel()
+        return mb._lanes(list(range(len(groups))), work)
 
     # pixels per group: 4*16, 2*64, 1*16, 3*16, 1*64
     groups = [(4, (4, 4)), (2, (8, 8)), (1, (4, 4)), (3, (4, 4)), (1, (8, 8))]
-    assert fake(1, groups)._lanes() == [[1, 0, 4, 3, 2]]
-    lanes = fake(2, groups)._lanes()
-    assert lanes == [[1, 3], [0, 4, 2]]
-    assert sorted(g for lane in lanes for g in lane) == list(range(5))
+    assert lanes(1, groups) == [[1, 0, 4, 3, 2]]
+    two = lanes(2, groups)
+    assert two == [[1, 3], [0, 4, 2]]
+    assert sorted(g for lane in two for g in lane) == list(range(5))
     # more streams than groups: one group per stream
-    assert fake(8, groups[:3])._lanes() == [[1], [0], [2]]
+    assert lanes(8, groups[:3]) == [[1], [0], [2]]
     with pytest.raises(ValueError):
         MixedBatch([], streams=0)

@@ -55,6 +55,10 @@ SIGNATURES = {
     'vmbd_ola_adjoint_cached': (ctypes.c_int, [_P, _I, _I, _P, _F, _F, _I, _P]),
     'vmbd_ola_forward_cached_ex': (ctypes.c_int, [_P, _I, _I, _P, _F, _F, _F, _F, _P]),
     'vmbd_ola_adjoint_cached_ex': (ctypes.c_int, [_P, _I, _I, _P, _F, _F, ctypes.c_float, _F, _I, _P]),
+    'vmbd_batch_create': (_P, [_P, _I, _P, _P, _P, _P, _P, _P, _P, _I]),
+    'vmbd_batch_destroy': (None, [_P]),
+    'vmbd_batch_launches': (ctypes.c_int, [_P]),
+    'vmbd_batch_step': (ctypes.c_int, [_P, _P]),
     'vmbd_deconv_workspace_bytes': (ctypes.c_size_t, [_P]),
     'vmbd_deconv_direct': (ctypes.c_int, [_P, _I, _P, _F, ctypes.c_float, ctypes.c_float, _F, _P]),
     'vmbd_deconv_gdirect': (ctypes.c_int, [_P, _F, _F, _F, _F, ctypes.c_float, ctypes.c_float, _F, _P, _P]),

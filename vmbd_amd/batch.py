@@ -5,31 +5,43 @@
     mb.step()                      # forward + both exact adjoints of every frame
     y_i, xbar_i, fbar_i = mb.outputs(i)
 
-Frames are grouped by geometry; each group is one plan whose launches carry
-all patches of all its frames (the kernels take the frame count as a grid
-dimension).  ``capture()`` records the launches of every group in one HIP
-graph, so a step of the whole mixed set is one graph replay; with
-``streams=k > 1`` (default 3) the graph forks the groups over k streams
-(largest groups first, each to the stream with the fewest pixels so far)
-and joins them at its end, so one group's launches fill the tail of
-another's (64 frames of six geometries: 2.48 ms per step with 3 streams,
-2.50 with 2, 2.60 with 1; 4 streams no faster than 3).  The eager
-``step()`` runs the groups in order on the current stream.  With
+Frames are grouped by geometry; each group is one plan over the resident
+stacks of its frames (the frame is a grid dimension of every pass).  Two
+launch forms:
+
+* ``grouped=True`` (default): the library's batch (vmbd_batch_*) runs each
+  pass of several groups as one launch over a device table of groups, one
+  launch per pass and distinct tile (the row, overlap-add and stack passes
+  are specialised to the tile width, the column passes to its height).  With
+  ``streams=1`` one batch holds every group (64 frames of six geometries on
+  three tiles: 19 launches per step instead of 42); with ``streams=k > 1``
+  the groups are split by tile into one batch per tile and ``capture()``
+  forks those batches over k streams (21 launches on three streams).
+* ``grouped=False``: the groups' own launches, seven per group; with
+  ``streams=k > 1`` (default 3) ``capture()`` forks the groups over k
+  streams inside the graph (largest groups first, each to the stream with
+  the fewest pixels so far) so that one group's launches fill the tail of
+  another's.
+
+``capture()`` records one step in a HIP graph and returns its replay.  With
 ``tune=True`` each group's FFT tile comes from the autotuner.
 """
+import ctypes
 from collections import OrderedDict
 
 import torch
 
+from . import _lib
 from .imagetools import win2winaux
 from .ola import OlaPlan
 
 
 class MixedBatch:
-    def __init__(self, specs, overlap=0.5, tune=False, exact=True, device=None, streams=3):
+    def __init__(self, specs, overlap=0.5, tune=False, exact=True, device=None, streams=3, grouped=True):
         if int(streams) < 1:
             raise ValueError(f'streams must be >= 1, got {streams}')
         self.streams = int(streams)
+        self.grouped = bool(grouped)
         self.device = torch.device(device) if device is not None else torch.device('cuda', torch.cuda.current_device())
         self.exact = bool(exact)
         self.specs = [(tuple(int(v) for v in sx), (int(c[0]), int(c[1])), (int(s[0]), int(s[1])))
@@ -59,6 +71,18 @@ class MixedBatch:
             for j, i in enumerate(idx):
                 self.where[i] = (gi, j)
         self._graph = None
+        self._vb = None
+
+    def __del__(self):
+        vbs = getattr(self, '_vb', None)
+        if vbs:
+            try:
+                torch.cuda.synchronize(self.device)
+                for vb in vbs.values():
+                    _lib.lib().vmbd_batch_destroy(vb)
+            except Exception:
+                pass
+            self._vb = None
 
     @property
     def pixels(self):
@@ -73,41 +97,102 @@ class MixedBatch:
             G['f'][j].copy_(fs[i])
             G['g'][j].copy_(gs[i])
 
+    # -- grouped launches -----------------------------------------------------
+    def _parts(self):
+        """Group indices per library batch: all groups in one batch on one
+        stream; split by tile when the step forks over several streams."""
+        if self.streams == 1:
+            return {None: list(range(len(self.groups)))}
+        parts = OrderedDict()
+        for gi, G in enumerate(self.groups):
+            parts.setdefault(G['plan'].tile, []).append(gi)
+        return parts
+
+    def _batches(self):
+        """The library batches bound to the group stacks (created once)."""
+        if self._vb is None:
+            L = _lib.lib()
+            self._vb = OrderedDict()
+            for key, gis in self._parts().items():
+                gs = [self.groups[gi] for gi in gis]
+                n = len(gs)
+                arr = lambda vals: (ctypes.c_void_p * n)(*vals)
+                ptrs = lambda k: arr([G[k].data_ptr() for G in gs])
+                nfr = (ctypes.c_int32 * n)(*[len(G['idx']) for G in gs])
+                with torch.cuda.device(self.device):
+                    h = L.vmbd_batch_create(arr([G['plan']._h.value for G in gs]), n, nfr, ptrs('x'), ptrs('f'),
+                                            ptrs('g'), ptrs('y'), ptrs('xb'), ptrs('fb'), int(self.exact))
+                if not h:
+                    raise _lib.VmbdError(f'vmbd_batch_create: {_lib.last_error()}', L.vmbd_last_status())
+                self._vb[key] = ctypes.c_void_p(h)
+        return self._vb
+
+    @property
+    def launches(self):
+        """Kernel launches per step."""
+        if self.grouped:
+            return sum(int(_lib.lib().vmbd_batch_launches(vb)) for vb in self._batches().values())
+        return 7 * len(self.groups)
+
+    def _batch_step(self, vb):
+        _lib.check(_lib.lib().vmbd_batch_step(vb, _lib.stream_ptr()), 'vmbd_batch_step')
+
+    def _mark_written(self):
+        for G in self.groups:
+            _lib.wrote(G['y'], G['xb'], G['fb'])
+            G['plan']._rows_key = None
+
+    # -- steps ----------------------------------------------------------------
     def _group_step(self, G):
         p = G['plan']
         p.forward(G['x'], G['f'], out=G['y'])
         p.adjoint_xf(G['g'], G['x'], G['f'], exact=self.exact, out_x=G['xb'], out_f=G['fb'])
 
     def step(self):
+        if self.grouped:
+            for vb in self._batches().values():
+                self._batch_step(vb)
+            self._mark_written()
+            return
         for G in self.groups:
             self._group_step(G)
 
-    def _lanes(self):
-        """Group indices per stream: largest groups first, each to the stream
-        with the fewest pixels so far."""
-        k = min(self.streams, len(self.groups))
+    def _lanes(self, units, work):
+        """Units per stream: largest first, each to the stream with the
+        fewest pixels so far."""
+        k = min(self.streams, len(units))
         load = [0] * k
         lanes = [[] for _ in range(k)]
-        work = lambda gi: len(self.groups[gi]['idx']) * self.groups[gi]['x'][0].numel()
-        for gi in sorted(range(len(self.groups)), key=work, reverse=True):
+        for u in sorted(units, key=work, reverse=True):
             q = load.index(min(load))
-            lanes[q].append(gi)
-            load[q] += work(gi)
+            lanes[q].append(u)
+            load[q] += work(u)
         return lanes
 
     def _forked_step(self, side):
-        """One step with the groups spread over the current stream and the
-        streams in side, which fork from and join back into the current one."""
+        """One step with the groups (or, grouped, the per-tile batches) spread
+        over the current stream and the streams in side, which fork from and
+        join back into the current one."""
         cur = torch.cuda.current_stream()
-        lanes = self._lanes()
+        gwork = lambda gi: len(self.groups[gi]['idx']) * self.groups[gi]['x'][0].numel()
+        if self.grouped:
+            parts = self._parts()
+            vbs = self._batches()
+            lanes = self._lanes(list(parts), lambda key: sum(gwork(gi) for gi in parts[key]))
+            run = lambda key: self._batch_step(vbs[key])
+        else:
+            lanes = self._lanes(list(range(len(self.groups))), gwork)
+            run = lambda gi: self._group_step(self.groups[gi])
         for ss in side[:len(lanes) - 1]:
             ss.wait_stream(cur)
         for q, lane in enumerate(lanes):
             with torch.cuda.stream(cur if q == 0 else side[q - 1]):
-                for gi in lane:
-                    self._group_step(self.groups[gi])
+                for u in lane:
+                    run(u)
         for ss in side[:len(lanes) - 1]:
             cur.wait_stream(ss)
+        if self.grouped:
+            self._mark_written()
 
     def capture(self):
         """Record one step of every group in a HIP graph; returns replay()."""

@@ -12,6 +12,7 @@
 #include "../../include/vmbd.h"
 #include "../../include/vmbd_solver.h"
 #include "capi_util.h"
+#include "col_device.h"
 #include "fft_core.h"
 #include "ola_kernels.h"
 
@@ -817,6 +818,161 @@ int vmbd_deconv_sparse_step(vmbd_plan* p, const void* spec, const void* ftpy, co
   d.clip_lo = 1;
   d.lo = 0.f;
   VMBD_LAUNCH(launch_rows_c2r_ola(g, d, p->tw, st));
+  return VMBD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batches of plans: the passes of every group in single launches
+// ---------------------------------------------------------------------------
+}  // extern "C"
+
+struct vmbd_batch {
+  int device = -1;
+  std::vector<vmbd_plan*> plans;
+  BatchGroup* dev = nullptr;
+  struct Launch {
+    int pass, tile, wgs;
+    size_t shm;
+    BatchLaunch L;
+  };
+  std::vector<Launch> launches;  // in pass order
+};
+
+namespace {
+
+// the step's parameters of one group: vmbd_ola_forward followed by
+// vmbd_ola_adjoint_xf reusing the forward's row transforms
+BatchGroup batch_group(vmbd_plan* p, int nframes, const float* x, const float* f, const float* gres, float* y,
+                       float* xbar, float* fbar, int exact) {
+  BatchGroup B{};
+  const Geo g = call_geo(p, nframes);
+  B.g = g;
+  B.tw = p->tw;
+  const RowSrc sx = src_image(p, x), sf = src_psf(p, f);
+  const int TPW = WG / (g.Tw / 16);
+  auto blocks = [&](const RowSrc& s) { return ((s.rhi - s.rlo + 1) / 2 + TPW - 1) / TPW; };
+  B.rows_fwd.n = 2;
+  B.rows_fwd.s[0] = sx;
+  B.rows_fwd.s[1] = sf;
+  B.rows_fwd.out[0] = p->wsX;
+  B.rows_fwd.out[1] = p->wsF;
+  B.rows_fwd.blk_end[0] = blocks(sx);
+  B.rows_fwd.blk_end[1] = B.rows_fwd.blk_end[0] + blocks(sf);
+  B.cf_a = raw_in(p->wsX, sx);
+  B.cf_b = raw_in(p->wsF, sf);
+  B.cf_o = ColOut{p->Y1, g.ext0};
+  B.ola_fwd = dst_forward(p, y);
+  const SharedRows sr = residual_rows(p, g, gres, exact);
+  B.gres = sr.g;
+  B.rows_res.n = 1;
+  B.rows_res.s[0] = sr.s;
+  B.rows_res.out[0] = p->wsG;
+  B.rows_res.blk_end[0] = blocks(sr.s);
+  B.ca_a = shared_in(p->wsG, sr);
+  B.ca_b = raw_in(p->wsF, sf);
+  B.ca_c = raw_in(p->wsX, sx);
+  B.ca_o1 = ColOut{p->Y1, g.sw0};
+  B.ca_o2 = ColOut{p->Y2, g.sf0};
+  B.ola_adj = dst_adjoint_x(p, xbar);
+  // the grouped OLA launch runs WG threads per workgroup for every group
+  B.st_src = p->Y2;
+  B.st_srow = g.sf0;
+  B.st_nrows = g.sf0;
+  B.st_ncols = g.sf1;
+  B.st_out = fbar;
+  B.st_scale = inv_scale(g);
+  return B;
+}
+
+void batch_free(vmbd_batch* b) {
+  if (!b) return;
+  (void)hipFree(b->dev);
+  delete b;
+}
+
+}  // namespace
+
+extern "C" {
+
+vmbd_batch* vmbd_batch_create(vmbd_plan* const* plans, int32_t n, const int32_t* nframes, const float* const* x,
+                              const float* const* f, const float* const* g, float* const* y, float* const* xbar,
+                              float* const* fbar, int32_t exact) {
+  vmbd_clear_error();
+  if (!plans || !nframes || !x || !f || !g || !y || !xbar || !fbar || n <= 0) {
+    vmbd_fail(VMBD_E_ARG, "vmbd_batch_create: null argument or n <= 0");
+    return nullptr;
+  }
+  if (n > BATCH_MAX_GROUPS) {
+    vmbd_fail(VMBD_E_UNSUP, "vmbd_batch_create: %d plans, at most %d", n, BATCH_MAX_GROUPS);
+    return nullptr;
+  }
+  std::vector<BatchGroup> host((size_t)n);
+  int dev = -1;
+  (void)hipGetDevice(&dev);
+  for (int i = 0; i < n; ++i) {
+    if (check_plan(plans[i], nframes[i])) return nullptr;
+    if (!x[i] || !f[i] || !g[i] || !y[i] || !xbar[i] || !fbar[i]) {
+      vmbd_fail(VMBD_E_ARG, "vmbd_batch_create: null buffer of plan %d", i);
+      return nullptr;
+    }
+    host[i] = batch_group(plans[i], nframes[i], x[i], f[i], g[i], y[i], xbar[i], fbar[i], exact);
+  }
+  vmbd_batch* b = new vmbd_batch();
+  b->device = dev;
+  b->plans.assign(plans, plans + n);
+  // per pass, one launch per distinct tile; groups in plan order
+  for (int pass = 0; pass < BP_N; ++pass) {
+    std::vector<int> tiles;
+    for (int i = 0; i < n; ++i) {
+      const int t = batch_tile(host[i], pass);
+      if (std::find(tiles.begin(), tiles.end(), t) == tiles.end()) tiles.push_back(t);
+    }
+    for (int t : tiles) {
+      vmbd_batch::Launch l{};
+      l.pass = pass;
+      l.tile = t;
+      int blk = 0;
+      for (int i = 0; i < n; ++i) {
+        if (batch_tile(host[i], pass) != t) continue;
+        l.L.gidx[l.L.ng] = i;
+        l.L.start[l.L.ng] = blk;
+        ++l.L.ng;
+        blk += (batch_blocks(host[i], pass) + 7) & ~7;
+        l.shm = std::max(l.shm, batch_shared_bytes(host[i], pass));
+      }
+      l.L.start[l.L.ng] = blk;
+      l.wgs = (pass == BP_OLA_FWD || pass == BP_OLA_ADJ) ? batch_ola_threads(host.data(), l.L, pass) : WG;
+      b->launches.push_back(l);
+    }
+  }
+  size_t tot = 0;
+  if (!dev_alloc(&b->dev, (size_t)n, tot)) {
+    batch_free(b);
+    vmbd_fail(VMBD_E_NOMEM, "vmbd_batch_create: hipMalloc failed");
+    return nullptr;
+  }
+  hipError_t e = hipMemcpy(b->dev, host.data(), (size_t)n * sizeof(BatchGroup), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    batch_free(b);
+    vmbd_fail_hip("vmbd_batch_create: hipMemcpy", e);
+    return nullptr;
+  }
+  return b;
+}
+
+void vmbd_batch_destroy(vmbd_batch* b) { batch_free(b); }
+
+int vmbd_batch_launches(const vmbd_batch* b) { return b ? (int)b->launches.size() : 0; }
+
+int vmbd_batch_step(vmbd_batch* b, vmbd_stream_t stream) {
+  vmbd_clear_error();
+  if (!b) return vmbd_fail(VMBD_E_ARG, "vmbd_batch_step: null batch");
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev != b->device)
+    return vmbd_fail(VMBD_E_ARG, "vmbd_batch_step: batch belongs to device %d, current device is %d", b->device, dev);
+  hipStream_t st = (hipStream_t)stream;
+  for (vmbd_plan* p : b->plans) p->rows_frames = 0;
+  for (const auto& l : b->launches) VMBD_LAUNCH(launch_batch(b->dev, l.L, l.pass, l.tile, l.wgs, l.shm, st));
   return VMBD_OK;
 }
 

@@ -151,6 +151,49 @@ hipError_t launch_cols_div(const Geo& g, int op, const ColIn* a, int na, DivPara
 // out = scale * inverse row transforms (+ beta * axz when axz != null)
 hipError_t launch_rows_c2r_stack(const Geo& g, const float2* src, int srow, float* out, int nrows, int ncols,
                                  float scale, const float2* tw, hipStream_t st, const float* axz = nullptr, float beta = 0.f);
+// Batches of several plans (frames of different geometry) whose passes run
+// as single launches over a table of groups (one group = one plan's frames):
+// everything the seven passes of a step (forward + both exact adjoints) need
+// for one group, built once when the batch binds its buffers.
+constexpr int BATCH_MAX_GROUPS = 16;
+struct BatchGroup {
+  Geo g;     // call geometry (P = frames * Pf)
+  Geo gres;  // geometry of the shared residual rows
+  RowJobs rows_fwd, rows_res;
+  ColIn cf_a, cf_b;
+  ColOut cf_o;
+  ColIn ca_a, ca_b, ca_c;
+  ColOut ca_o1, ca_o2;
+  OlaDst ola_fwd, ola_adj;
+  const float2* st_src;
+  int st_srow, st_nrows, st_ncols;
+  float* st_out;
+  float st_scale;
+  const float2* tw;
+};
+enum BatchPass { BP_ROWS_FWD = 0, BP_COLS_FWD, BP_OLA_FWD, BP_ROWS_RES, BP_COLS_ADJ, BP_OLA_ADJ, BP_STACK, BP_N };
+// One launch of a pass over ng groups of the same tile (Tw for the row, OLA
+// and stack passes, Th for the column passes): launch group gi is
+// groups[gidx[gi]] and owns blocks [start[gi], start[gi+1]); every start is a
+// multiple of 8, so the XCD a block lands on is the same as in a launch of
+// its group alone.
+struct BatchLaunch {
+  int ng;
+  int gidx[BATCH_MAX_GROUPS];
+  int start[BATCH_MAX_GROUPS + 1];
+};
+// blocks the group needs in the pass (before rounding up to 8)
+int batch_blocks(const BatchGroup& G, int pass);
+// the pass's tile (template parameter) of the group
+int batch_tile(const BatchGroup& G, int pass);
+// LDS bytes of the pass's launch for the group (dynamic, OLA passes only)
+size_t batch_shared_bytes(const BatchGroup& G, int pass);
+// threads per workgroup of an OLA pass over the launch's groups: 128 or 192
+// when every group's row fits one round of that many threads, else 256
+int batch_ola_threads(const BatchGroup* host_groups, const BatchLaunch& L, int pass);
+hipError_t launch_batch(const BatchGroup* dev_groups, const BatchLaunch& L, int pass, int tile, int wgs, size_t shm,
+                        hipStream_t st);
+
 size_t ola_rowbuf_bytes(const Geo& g, const OlaDst& d);
 size_t ola_shared_bytes(const Geo& g, const OlaDst& d);
 int ola_single_round(const Geo& g, const OlaDst& d);

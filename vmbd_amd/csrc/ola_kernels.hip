@@ -26,6 +26,8 @@
 // LDS: each workgroup owns one buffer; the FFT slices and the staging area of
 // the transposed store alias it (separated by workgroup barriers), which keeps
 // the footprint at 35-40 KB and allows four workgroups per CU.
+#include <algorithm>
+
 #include "col_device.h"
 #include "fft_core.h"
 #include "ola_kernels.h"
@@ -38,42 +40,23 @@ namespace vmbd {
 // ----------------------------------------------------------------------------
 // rows_r2c
 // ----------------------------------------------------------------------------
+// LDS (float2) of one rows_r2c workgroup: the transforms' slices, aliased by
+// the staging of the bin-major store
 template <int TW>
-__global__ __launch_bounds__(WG) void k_rows_r2c(Geo g, RowJobs jobs, const float2* __restrict__ tw) {
+struct RowsLds {
+  static constexpr int TPW = WG / (TW / 16);
+  static constexpr int v = cmax<TPW * FftLds<TW>::floats2, 2 * TPW * (TW / 2 + 1)>::v;
+};
+
+// One workgroup of the row pass: row pairs [q0, q0 + TPW) of patch pu of
+// source s into out (sm: RowsLds<TW>::v float2 of LDS).
+template <int TW>
+__device__ __forceinline__ void rows_r2c_body(Geo g, RowSrc s, float2* __restrict__ out, int pu, int q0, float2* sm,
+                                              const float2* __restrict__ tw) {
   constexpr int TPT = TW / 16;
   constexpr int TPW = WG / TPT;
   constexpr int NBN = TW / 2 + 1;
   constexpr int SL = FftLds<TW>::floats2;
-  __shared__ float2 sm[cmax<TPW * SL, 2 * TPW * NBN>::v];
-
-  // 1-D grid over work units (patch, row block), row blocks of all jobs
-  // job-major.  Workgroup ids are dealt round-robin to the NXCD XCDs; XCD j
-  // runs the contiguous units [start_j, start_j + count_j).  Units are ordered
-  // in tiles of TP = ceil(P / NXCD) consecutive patches (one XCD's share),
-  // row-block-major inside a tile, so neighbouring patches, which read the
-  // same pixels, run on one XCD at the same time and share its L2.  The map
-  // id -> (patch, row block) is a bijection whatever the hardware placement.
-  const int nblk = jobs.blk_end[jobs.n - 1];
-  const int total = g.P * nblk;
-  const int xcd = (int)blockIdx.x % NXCD;
-  const int u = xcd * (total / NXCD) + min(xcd, total % NXCD) + (int)blockIdx.x / NXCD;
-  const int TP = (g.P + NXCD - 1) / NXCD;
-  const int tile = u / (TP * nblk);
-  const int ur = u - tile * TP * nblk;
-  const int Tt = min(TP, g.P - tile * TP);
-  const int by = ur / Tt;
-  const int pu = tile * TP + (ur - by * Tt);
-  int ji = 0;
-  int blk0 = 0;
-#pragma unroll
-  for (int jj = 0; jj < ROW_JOBS_MAX - 1; ++jj)
-    if (jobs.n > jj + 1 && by >= jobs.blk_end[jj]) {
-      ji = jj + 1;
-      blk0 = jobs.blk_end[jj];
-    }
-  const RowSrc s = jobs.s[ji];
-  float2* __restrict__ out = jobs.out[ji];
-  const int q0 = (by - blk0) * TPW;  // first row pair of the block
 
   const int tid = threadIdx.x;
   const int tr = tid / TPT;
@@ -211,6 +194,49 @@ __global__ __launch_bounds__(WG) void k_rows_r2c(Geo g, RowJobs jobs, const floa
     }
 }
 
+// 1-D grid over work units (patch, row block), row blocks of all jobs
+// job-major.  Workgroup ids are dealt round-robin to the NXCD XCDs; XCD j
+// runs the contiguous units [start_j, start_j + count_j).  Units are ordered
+// in tiles of TP = ceil(P / NXCD) consecutive patches (one XCD's share),
+// row-block-major inside a tile, so neighbouring patches, which read the
+// same pixels, run on one XCD at the same time and share its L2.  The map
+// id -> (patch, row block) is a bijection whatever the hardware placement.
+// blk: the workgroup's id among the launch's g.P * (row blocks) (its XCD is
+// blk mod NXCD).  Returns the patch, the job and its first row pair.
+template <int TW>
+__device__ __forceinline__ void rows_unit(const Geo& g, const RowJobs& jobs, int blk, int& pu, int& ji, int& q0) {
+  constexpr int TPW = WG / (TW / 16);
+  const int nblk = jobs.blk_end[jobs.n - 1];
+  const int total = g.P * nblk;
+  const int xcd = blk % NXCD;
+  const int u = xcd * (total / NXCD) + min(xcd, total % NXCD) + blk / NXCD;
+  const int TP = (g.P + NXCD - 1) / NXCD;
+  const int tile = u / (TP * nblk);
+  const int ur = u - tile * TP * nblk;
+  const int Tt = min(TP, g.P - tile * TP);
+  const int by = ur / Tt;
+  pu = tile * TP + (ur - by * Tt);
+  ji = 0;
+  int blk0 = 0;
+#pragma unroll
+  for (int jj = 0; jj < ROW_JOBS_MAX - 1; ++jj)
+    if (jobs.n > jj + 1 && by >= jobs.blk_end[jj]) {
+      ji = jj + 1;
+      blk0 = jobs.blk_end[jj];
+    }
+  q0 = (by - blk0) * TPW;
+}
+
+template <int TW>
+__global__ __launch_bounds__(WG) void k_rows_r2c(Geo g, RowJobs jobs, const float2* __restrict__ tw) {
+  __shared__ float2 sm[RowsLds<TW>::v];
+
+  int pu, ji, q0;
+  rows_unit<TW>(g, jobs, (int)blockIdx.x, pu, ji, q0);
+  const RowSrc s = jobs.s[ji];
+  rows_r2c_body<TW>(g, s, jobs.out[ji], pu, q0, sm, tw);
+}
+
 template <int TW>
 static hipError_t rows_r2c_t(const Geo& g, const RowSrc* s, float2* const* out, int n, const float2* tw,
                              hipStream_t st) {
@@ -325,22 +351,27 @@ hipError_t launch_rows_r2c(const Geo& g, const RowSrc& s, float2* out, const flo
     }                                                                                               \
   } while (0)
 
-// 256- and 512-point columns: four wavefronts per SIMD (at most 128 VGPRs,
-// a few spilled at 512) beat three without spills; the LDS allows four
-// workgroups per CU
+// LDS (float2) of one column-pass workgroup: the transforms' slices, aliased
+// by the row staging of the store
 template <int TH, int OP>
-__global__ __launch_bounds__(WG, (TH >= 256 ? 4 : 1)) void k_cols(Geo g, ColIn a, ColIn b, ColIn c3, ColOut o1, ColOut o2,
-                                             float2* __restrict__ spec_out, const float2* __restrict__ tw) {
+struct ColsLds {
+  static constexpr int CS = WG / (TH / 16);
+  static constexpr int v = (OP == COL_SPEC) ? CS * FftLds<TH>::floats2 : cmax<CS * FftLds<TH>::floats2, TH * (CS + 1)>::v;
+};
+
+// One workgroup of the column pass: strip `strip` of patch p (sm:
+// ColsLds<TH, OP>::v float2 of LDS).
+template <int TH, int OP>
+// (parameters by value: with references the compiler keeps the column
+// descriptors in vector registers and spills at 256 points)
+__device__ __forceinline__ void cols_body(Geo g, ColIn a, ColIn b, ColIn c3, ColOut o1, ColOut o2,
+                                          float2* __restrict__ spec_out, int strip, int p, float2* sm,
+                                          const float2* __restrict__ tw) {
   constexpr int TPT = TH / 16;
   constexpr int CS = WG / TPT;  // columns per workgroup
   constexpr int SL = FftLds<TH>::floats2;
-  __shared__ float2 sm[(OP == COL_SPEC) ? CS * SL : cmax<CS * SL, TH * (CS + 1)>::v];
 
   const int NB = g.NB;
-  // strips of one patch are consecutive workgroups; each writes whole
-  // 128-byte lines of its rows (16 bins, row pitch NBP a multiple of 16)
-  const int strip = blockIdx.x;
-  const int p = (int)blockIdx.z * g.Pf + (int)blockIdx.y;  // frame blockIdx.z, patch blockIdx.y
   const int cc = threadIdx.x / TPT;
   const int t = threadIdx.x - cc * TPT;
   const int c = strip * CS + cc;
@@ -426,6 +457,19 @@ __global__ __launch_bounds__(WG, (TH >= 256 ? 4 : 1)) void k_cols(Geo g, ColIn a
         col_store_rows<TH, CS, true>(B, sm, o2, p, strip, NB, g.NBP, cc, t);
     }
   }
+}
+
+// 256- and 512-point columns: four wavefronts per SIMD (at most 128 VGPRs,
+// a few spilled at 512) beat three without spills; the LDS allows four
+// workgroups per CU.  Strips of one patch are consecutive workgroups; each
+// writes whole 128-byte lines of its rows (16 bins, row pitch NBP a multiple
+// of 16).
+template <int TH, int OP>
+__global__ __launch_bounds__(WG, (TH >= 256 ? 4 : 1)) void k_cols(Geo g, ColIn a, ColIn b, ColIn c3, ColOut o1, ColOut o2,
+                                             float2* __restrict__ spec_out, const float2* __restrict__ tw) {
+  __shared__ float2 sm[ColsLds<TH, OP>::v];
+  // frame blockIdx.z, patch blockIdx.y
+  cols_body<TH, OP>(g, a, b, c3, o1, o2, spec_out, blockIdx.x, (int)blockIdx.z * g.Pf + (int)blockIdx.y, sm, tw);
 }
 
 #undef VMBD_COL_PRODUCT
@@ -613,18 +657,18 @@ constexpr int OLA_KJ = 8;  // output columns per thread of streamed launches (O1
 // forward's code carries no window path.
 // WGS: threads per workgroup (ola_wg): single-round launches of a full
 // grid take only the threads their transforms need.
+// One workgroup of the OLA pass: output row i of frame fr (dyn: the
+// launch's ola_shared_bytes of LDS).
 template <int TW, bool STREAM, bool WIN, int WGS>
-__global__ __launch_bounds__(WGS, (TW >= 512 ? 3 : 1)) void k_rows_c2r_ola(Geo g, OlaDst d, const float2* __restrict__ tw) {
+__device__ __forceinline__ void c2r_ola_body(Geo g, OlaDst d, int i, int fr, float2* dyn,
+                                             const float2* __restrict__ tw) {
   constexpr int TPT = TW / 16;
   constexpr int wgs = WGS;
   constexpr int TPW = WGS / TPT;
   constexpr int SL = FftLds<TW>::floats2;
-  extern __shared__ float2 dyn[];
   float2* fl = dyn;
   float* rowbuf = (STREAM || d.alias) ? (float*)dyn : (float*)(dyn + TPW * SL);
 
-  const int fr = blockIdx.y;
-  const int i = blockIdx.x;
   const int u = i + d.oc0;
   const int kf = fr * g.Pf;  // first patch of this frame
   // covering patch rows m: m*hop0 <= u < m*hop0 + ext0
@@ -784,6 +828,12 @@ __global__ __launch_bounds__(WGS, (TW >= 512 ? 3 : 1)) void k_rows_c2r_ola(Geo g
   }
 }
 
+template <int TW, bool STREAM, bool WIN, int WGS>
+__global__ __launch_bounds__(WGS, (TW >= 512 ? 3 : 1)) void k_rows_c2r_ola(Geo g, OlaDst d, const float2* __restrict__ tw) {
+  extern __shared__ float2 dyn[];
+  c2r_ola_body<TW, STREAM, WIN, WGS>(g, d, blockIdx.x, blockIdx.y, dyn, tw);
+}
+
 // float offset of the staged windows: past the FFT slices and the gather
 // buffer (which alias each other in single-round launches), rounded up to
 // 16 bytes for the windows' 16-byte loads
@@ -877,22 +927,27 @@ hipError_t launch_rows_c2r_ola(const Geo& g, const OlaDst& d, const float2* tw, 
 // rows_c2r_stack: (P, nrows, ncols) stack from the first rows/cols of each tile
 // ----------------------------------------------------------------------------
 template <int TW>
-__global__ __launch_bounds__(WG) void k_rows_c2r_stack(Geo g, const float2* __restrict__ src, int srow,
-                                                       float* __restrict__ out, int nrows, int ncols, float scale,
-                                                       const float2* __restrict__ tw, const float* __restrict__ axz,
-                                                       float beta) {
+struct StackLds {
+  static constexpr int v = (WG / (TW / 16)) * FftLds<TW>::floats2;
+};
+
+// One workgroup of the stack pass: row block by of patch p (fl:
+// StackLds<TW>::v float2 of LDS).
+template <int TW>
+__device__ __forceinline__ void c2r_stack_body(Geo g, const float2* __restrict__ src, int srow,
+                                               float* __restrict__ out, int nrows, int ncols, float scale,
+                                               const float2* __restrict__ tw, const float* __restrict__ axz, float beta,
+                                               int p, int by, float2* fl) {
   constexpr int TPT = TW / 16;
   constexpr int TPW = WG / TPT;
   constexpr int SL = FftLds<TW>::floats2;
-  __shared__ float2 fl[TPW * SL];
   const int tr = threadIdx.x / TPT;
   const int t = threadIdx.x - tr * TPT;
   const int NQ = (nrows + 1) >> 1;
-  const int q = blockIdx.y * TPW + tr;
+  const int q = by * TPW + tr;
   const bool act = q < NQ;
   // no workgroup barrier follows: a wavefront without row pairs is done
-  if (blockIdx.y * TPW + (int)(threadIdx.x >> 6) * (64 / TPT) >= NQ) return;
-  const int p = blockIdx.x;
+  if (by * TPW + (int)(threadIdx.x >> 6) * (64 / TPT) >= NQ) return;
   const int r0 = 2 * q, r1 = 2 * q + 1;
   const float2* ra = act ? src + ((size_t)p * srow + r0) * g.NBP : src;
   const bool hasb = act && r1 < nrows;
@@ -921,6 +976,15 @@ __global__ __launch_bounds__(WG) void k_rows_c2r_stack(Geo g, const float2* __re
 }
 
 template <int TW>
+__global__ __launch_bounds__(WG) void k_rows_c2r_stack(Geo g, const float2* __restrict__ src, int srow,
+                                                       float* __restrict__ out, int nrows, int ncols, float scale,
+                                                       const float2* __restrict__ tw, const float* __restrict__ axz,
+                                                       float beta) {
+  __shared__ float2 fl[StackLds<TW>::v];
+  c2r_stack_body<TW>(g, src, srow, out, nrows, ncols, scale, tw, axz, beta, blockIdx.x, blockIdx.y, fl);
+}
+
+template <int TW>
 static hipError_t c2r_stack_t(const Geo& g, const float2* src, int srow, float* out, int nrows, int ncols,
                               float scale, const float2* tw, hipStream_t st, const float* axz, float beta) {
   constexpr int TPW = WG / (TW / 16);
@@ -942,6 +1006,169 @@ hipError_t launch_rows_c2r_stack(const Geo& g, const float2* src, int srow, floa
     case 128: return c2r_stack_t<128>(g, src, srow, out, nrows, ncols, scale, tw, st, axz, beta);
     case 256: return c2r_stack_t<256>(g, src, srow, out, nrows, ncols, scale, tw, st, axz, beta);
     case 512: return c2r_stack_t<512>(g, src, srow, out, nrows, ncols, scale, tw, st, axz, beta);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+// ----------------------------------------------------------------------------
+// batches: one launch per pass over the groups of a table
+// ----------------------------------------------------------------------------
+// Each kernel finds its group from the launch's block ranges (a scalar scan
+// of at most BATCH_MAX_GROUPS starts), takes that group's parameters from the
+// table in device memory and runs the same workgroup body as the single-plan
+// kernel with the block index local to the group.  Blocks past a group's
+// count (its range is rounded up to a multiple of 8) exit at once.
+__device__ __forceinline__ int batch_find(const BatchLaunch& L, int b) {
+  int gi = 0;
+  for (int j = 1; j < L.ng; ++j)
+    if (b >= L.start[j]) gi = j;
+  return gi;
+}
+
+template <int TW>
+__global__ __launch_bounds__(WG) void k_batch_rows(const BatchGroup* __restrict__ G, BatchLaunch L, int res) {
+  __shared__ float2 sm[RowsLds<TW>::v];
+  const int gi = batch_find(L, (int)blockIdx.x);
+  const int lb = (int)blockIdx.x - L.start[gi];
+  const BatchGroup& B = G[L.gidx[gi]];
+  const Geo& g = res ? B.gres : B.g;
+  const RowJobs& jobs = res ? B.rows_res : B.rows_fwd;
+  if (lb >= g.P * jobs.blk_end[jobs.n - 1]) return;
+  int pu, ji, q0;
+  rows_unit<TW>(g, jobs, lb, pu, ji, q0);
+  const RowSrc s = jobs.s[ji];
+  rows_r2c_body<TW>(g, s, jobs.out[ji], pu, q0, sm, B.tw);
+}
+
+template <int TH, int OP>
+__global__ __launch_bounds__(WG, (TH >= 256 ? 4 : 1)) void k_batch_cols(const BatchGroup* __restrict__ G,
+                                                                        BatchLaunch L) {
+  constexpr int CS = WG / (TH / 16);
+  __shared__ float2 sm[ColsLds<TH, OP>::v];
+  const int gi = batch_find(L, (int)blockIdx.x);
+  const int lb = (int)blockIdx.x - L.start[gi];
+  const BatchGroup& B = G[L.gidx[gi]];
+  const int strips = (B.g.NB - 1 + CS - 1) / CS;
+  const int p = lb / strips;
+  if (p >= B.g.P) return;
+  if constexpr (OP == COL_FWD)
+    cols_body<TH, COL_FWD>(B.g, B.cf_a, B.cf_b, B.cf_b, B.cf_o, B.cf_o, nullptr, lb - p * strips, p, sm, B.tw);
+  else
+    cols_body<TH, COL_ADJ2>(B.g, B.ca_a, B.ca_b, B.ca_c, B.ca_o1, B.ca_o2, nullptr, lb - p * strips, p, sm, B.tw);
+}
+
+// WGS < WG: every group of the launch holds its row in one round of WGS
+// threads (batch_ola_threads), so only the single-round body is needed
+template <int TW, bool WIN, int WGS>
+__global__ __launch_bounds__(WGS, (TW >= 512 ? 3 : 1)) void k_batch_ola(const BatchGroup* __restrict__ G,
+                                                                        BatchLaunch L) {
+  extern __shared__ float2 dyn[];
+  const int gi = batch_find(L, (int)blockIdx.x);
+  const int lb = (int)blockIdx.x - L.start[gi];
+  const BatchGroup& B = G[L.gidx[gi]];
+  const OlaDst& d = WIN ? B.ola_adj : B.ola_fwd;
+  const int fr = lb / d.O0;
+  if (fr >= B.g.P / B.g.Pf) return;
+  if constexpr (WGS == WG) {
+    if (d.stream) {
+      c2r_ola_body<TW, true, WIN, WG>(B.g, d, lb - fr * d.O0, fr, dyn, B.tw);
+      return;
+    }
+  }
+  c2r_ola_body<TW, false, WIN, WGS>(B.g, d, lb - fr * d.O0, fr, dyn, B.tw);
+}
+
+template <int TW>
+__global__ __launch_bounds__(WG) void k_batch_stack(const BatchGroup* __restrict__ G, BatchLaunch L) {
+  constexpr int TPW = WG / (TW / 16);
+  __shared__ float2 fl[StackLds<TW>::v];
+  const int gi = batch_find(L, (int)blockIdx.x);
+  const int lb = (int)blockIdx.x - L.start[gi];
+  const BatchGroup& B = G[L.gidx[gi]];
+  const int by = lb / B.g.P;
+  if (by * TPW >= (B.st_nrows + 1) / 2) return;
+  c2r_stack_body<TW>(B.g, B.st_src, B.st_srow, B.st_out, B.st_nrows, B.st_ncols, B.st_scale, B.tw, nullptr, 0.f,
+                     lb - by * B.g.P, by, fl);
+}
+
+int batch_tile(const BatchGroup& G, int pass) {
+  return (pass == BP_COLS_FWD || pass == BP_COLS_ADJ) ? G.g.Th : G.g.Tw;
+}
+
+int batch_blocks(const BatchGroup& G, int pass) {
+  const int TPW = WG / (G.g.Tw / 16);
+  switch (pass) {
+    case BP_ROWS_FWD: return G.g.P * G.rows_fwd.blk_end[G.rows_fwd.n - 1];
+    case BP_ROWS_RES: return G.gres.P * G.rows_res.blk_end[G.rows_res.n - 1];
+    case BP_COLS_FWD:
+    case BP_COLS_ADJ: {
+      const int CS = WG / (G.g.Th / 16);
+      return G.g.P * ((G.g.NB - 1 + CS - 1) / CS);
+    }
+    case BP_OLA_FWD: return G.ola_fwd.O0 * (G.g.P / G.g.Pf);
+    case BP_OLA_ADJ: return G.ola_adj.O0 * (G.g.P / G.g.Pf);
+    case BP_STACK: return G.g.P * (((G.st_nrows + 1) / 2 + TPW - 1) / TPW);
+    default: return 0;
+  }
+}
+
+size_t batch_shared_bytes(const BatchGroup& G, int pass) {
+  if (pass == BP_OLA_FWD) return ola_shared_bytes(G.g, G.ola_fwd);
+  if (pass == BP_OLA_ADJ) return ola_shared_bytes(G.g, G.ola_adj);
+  return 0;
+}
+
+int batch_ola_threads(const BatchGroup* H, const BatchLaunch& L, int pass) {
+  int need = 0;
+  for (int i = 0; i < L.ng; ++i) {
+    const BatchGroup& B = H[L.gidx[i]];
+    const OlaDst& d = pass == BP_OLA_ADJ ? B.ola_adj : B.ola_fwd;
+    if (!d.alias) return WG;
+    need = std::max(need, (B.g.Tw / 16) * d.mc * ((B.g.csf1 + 1) / 2));
+  }
+  return need <= 128 ? 128 : need <= 192 ? 192 : WG;
+}
+
+template <int T, bool WIN>
+static void batch_ola_launch(const BatchGroup* G, const BatchLaunch& L, int wgs, size_t shm, hipStream_t st,
+                             dim3 grid) {
+  switch (wgs) {
+    case 128: hipLaunchKernelGGL((k_batch_ola<T, WIN, 128>), grid, dim3(128), shm, st, G, L); break;
+    case 192: hipLaunchKernelGGL((k_batch_ola<T, WIN, 192>), grid, dim3(192), shm, st, G, L); break;
+    default: hipLaunchKernelGGL((k_batch_ola<T, WIN, WG>), grid, dim3(WG), shm, st, G, L); break;
+  }
+}
+
+template <int T>
+static hipError_t batch_t(const BatchGroup* G, const BatchLaunch& L, int pass, int wgs, size_t shm,
+                          hipStream_t st) {
+  const dim3 grid((unsigned)L.start[L.ng]);
+  if (grid.x == 0) return hipSuccess;
+  static const char* names[BP_N] = {"batch_rows_fwd", "batch_cols_fwd", "batch_ola_fwd", "batch_rows_res",
+                                    "batch_cols_adj2", "batch_ola_win", "batch_stack"};
+  ProfScope ps(names[pass], st);
+  switch (pass) {
+    case BP_ROWS_FWD: hipLaunchKernelGGL(k_batch_rows<T>, grid, dim3(WG), 0, st, G, L, 0); break;
+    case BP_ROWS_RES: hipLaunchKernelGGL(k_batch_rows<T>, grid, dim3(WG), 0, st, G, L, 1); break;
+    case BP_COLS_FWD: hipLaunchKernelGGL((k_batch_cols<T, COL_FWD>), grid, dim3(WG), 0, st, G, L); break;
+    case BP_COLS_ADJ: hipLaunchKernelGGL((k_batch_cols<T, COL_ADJ2>), grid, dim3(WG), 0, st, G, L); break;
+    case BP_OLA_FWD: batch_ola_launch<T, false>(G, L, wgs, shm, st, grid); break;
+    case BP_OLA_ADJ: batch_ola_launch<T, true>(G, L, wgs, shm, st, grid); break;
+    case BP_STACK: hipLaunchKernelGGL(k_batch_stack<T>, grid, dim3(WG), 0, st, G, L); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_batch(const BatchGroup* dev_groups, const BatchLaunch& L, int pass, int tile, int wgs, size_t shm,
+                        hipStream_t st) {
+  switch (tile) {
+    case 16: return batch_t<16>(dev_groups, L, pass, wgs, shm, st);
+    case 32: return batch_t<32>(dev_groups, L, pass, wgs, shm, st);
+    case 64: return batch_t<64>(dev_groups, L, pass, wgs, shm, st);
+    case 128: return batch_t<128>(dev_groups, L, pass, wgs, shm, st);
+    case 256: return batch_t<256>(dev_groups, L, pass, wgs, shm, st);
+    case 512: return batch_t<512>(dev_groups, L, pass, wgs, shm, st);
     default: return hipErrorInvalidValue;
   }
 }
