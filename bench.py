@@ -200,7 +200,7 @@ def plain_form(wa, x, f, g, timed, a):
                     'operation, complex (T x T) spectra, patch-local transposes'}
 
 
-def mfbd_rate(sx, csf, sf, walk, nframes, timed, iters=20, prior=None):
+def mfbd_rate(sx, csf, sf, walk, nframes, timed, iters=20, prior=None, tune=True):
     """Online MFBD frames/s: latent sx, PSF grid csf of sf PSFs, per frame
     `iters` PBB f-iterations + `iters` PBB x-iterations (exact adjoints) and
     the optional prior pass, captured in a HIP graph and replayed per frame."""
@@ -224,7 +224,7 @@ def mfbd_rate(sx, csf, sf, walk, nframes, timed, iters=20, prior=None):
         noise = torch.as_tensor((0.01 * rng.standard_normal(plan.sy)).astype(np.float32), device=dev)
         obs.append(yt + noise)
     m = OnlineMFBD(obs[0], sf, csf, f_solver='pbb', x_solver='pbb', f_iters=iters, x_iters=iters, f_beta=1e-3,
-                   exact=True, capture=True, prior=prior, device=dev)
+                   exact=True, capture=True, prior=prior, device=dev, tune=tune)
     for i in range(2):
         m.step(obs[i % 4])
     k = [0]
@@ -235,8 +235,9 @@ def mfbd_rate(sx, csf, sf, walk, nframes, timed, iters=20, prior=None):
 
     ms = timed(one, nframes)
     return {'frames_per_s': round(1e3 / ms, 2), 'ms_per_frame': round(ms, 3), 'frames': nframes,
-            'latent': list(sx), 'csf': list(csf), 'sf': list(sf), 'tile': list(plan.tile), 'f_iters': iters,
-            'x_iters': iters, 'prior': prior, 'graph': True}
+            'latent': list(sx), 'csf': list(csf), 'sf': list(sf), 'tile': list(m.plan.tile),
+            'launch_shape': m.plan.tuning, 'autotuned': bool(tune), 'f_iters': iters, 'x_iters': iters,
+            'prior': prior, 'graph': True}
 
 
 C4_GEOMS = [((512, 512), (12, 12), (15, 15)), ((512, 512), (6, 6), (41, 41)), ((768, 1024), (8, 8), (25, 25)),

@@ -74,6 +74,21 @@ typedef struct vmbd_geom {
 
 typedef struct vmbd_plan vmbd_plan;
 
+/* Launch shapes of a plan's passes (no counterpart in the reference, whose
+ * kernels have fixed 16x16 / 32x32 blocks, gputools.py:372-401); 0 =
+ * automatic.  Any setting gives the same results (bit for bit): it only
+ * moves work between workgroups.
+ *   ola_threads  threads per workgroup of the overlap-add passes: 128, 192
+ *                or 256 (one output row per workgroup; 128 and 192 need
+ *                every row's transforms in one round of that many threads)
+ *   col_threads  threads per workgroup of the column passes: 128 or 256
+ *                (a workgroup carries 16 * threads / Th columns of one
+ *                patch, i.e. threads / 64 wavefronts per column strip) */
+typedef struct vmbd_tuning {
+  int32_t ola_threads;
+  int32_t col_threads;
+} vmbd_tuning;
+
 /* Library version (major*10000 + minor*100 + patch). */
 int vmbd_version(void);
 
@@ -93,6 +108,11 @@ int vmbd_last_status(void);
 vmbd_plan* vmbd_plan_create(const vmbd_geom* geom, const float* win_table, int32_t n_win,
                             const int32_t* win_idx);
 void vmbd_plan_destroy(vmbd_plan* plan);
+
+/* Set / read the plan's launch shapes (VMBD_E_UNSUP for a shape the
+ * plan's geometry cannot use). */
+int vmbd_plan_set_tuning(vmbd_plan* plan, const vmbd_tuning* tuning);
+int vmbd_plan_get_tuning(const vmbd_plan* plan, vmbd_tuning* tuning);
 
 /* Device bytes held by the plan (workspaces, windows, twiddles). */
 size_t vmbd_plan_workspace_bytes(const vmbd_plan* plan);

@@ -108,3 +108,43 @@ def test_autotuned_tile_gives_same_result(cuda, tmp_path, monkeypatch):
     for tl in autotune.candidates(np.broadcast_to(wa.sw, (2,)), sf):
         p = OlaPlan(sx, sf, wa, 'valid', tile=tl)
         assert rel_l2(p.forward(t(x), t(f)).cpu().numpy(), O.forward(x64, f64, owa)) < 1e-5, tl
+
+
+def test_launch_shapes_same_bits_and_tuner(cuda, tmp_path, monkeypatch):
+    """Every launch shape a plan accepts (overlap-add and column-pass
+    workgroup sizes) gives the same bits; shapes a geometry cannot use are
+    refused; the tuner's search covers tile x shape and its choice is one of
+    the accepted configurations."""
+    import torch
+    from vmbd_amd import _lib, autotune
+    from vmbd_amd.imagetools import win2winaux
+    from vmbd_amd.ola import OlaPlan
+    print('case launch shapes', flush=True)
+    monkeypatch.setattr(autotune, 'CACHE', str(tmp_path / 'tiles.json'))
+    t = lambda a: torch.as_tensor(np.asarray(a, dtype=np.float32), device=cuda)
+    for sx, csf, sf in (((256, 256), (4, 4), (15, 15)), ((200, 400), (4, 24), (9, 9))):
+        wa = win2winaux(sx, csf, 0.5)
+        x, f, g = case_inputs(71, sx, csf, sf)
+        ref = None
+        accepted = 0
+        for shape in autotune.SHAPES:
+            p = OlaPlan(sx, sf, wa, 'valid')
+            try:
+                p.set_tuning(*shape)
+            except _lib.VmbdError as e:
+                assert e.code == _lib.VMBD_E_UNSUP and shape[0] in (128, 192), (shape, e)
+                continue
+            assert p.tuning == dict(ola_threads=shape[0], col_threads=shape[1])
+            out = (p.forward(t(x), t(f)),) + p.adjoint_xf(t(g), t(x), t(f))
+            if ref is None:
+                ref = out
+            assert all(torch.equal(a, b) for a, b in zip(out, ref)), (sx, shape)
+            accepted += 1
+        assert accepted >= 4, sx
+    sx, csf, sf = (256, 256), (4, 4), (15, 15)
+    wa = win2winaux(sx, csf, 0.5)
+    cfg, times = autotune.best_config(sx, sf, wa, frames=2)
+    assert len(times) >= 8
+    assert autotune.best_config(sx, sf, wa, frames=2)[0] == cfg          # cached
+    p = OlaPlan(sx, sf, wa, 'valid', tile=cfg['tile'])
+    p.set_tuning(cfg['ola_threads'], cfg['col_threads'])

@@ -21,6 +21,10 @@ CACHE_IMAGE = 1
 REUSE_ROWS = 1
 
 
+class VmbdTuning(ctypes.Structure):
+    _fields_ = [('ola_threads', ctypes.c_int32), ('col_threads', ctypes.c_int32)]
+
+
 class VmbdGeom(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int32) for n in (
         'H', 'W', 'sfh', 'sfw', 'csf0', 'csf1', 'sw0', 'sw1',
@@ -40,6 +44,8 @@ SIGNATURES = {
     'vmbd_last_status': (ctypes.c_int, []),
     'vmbd_plan_create': (_P, [ctypes.POINTER(VmbdGeom), ctypes.c_void_p, _I, ctypes.c_void_p]),
     'vmbd_plan_destroy': (None, [_P]),
+    'vmbd_plan_set_tuning': (ctypes.c_int, [_P, ctypes.POINTER(VmbdTuning)]),
+    'vmbd_plan_get_tuning': (ctypes.c_int, [_P, ctypes.POINTER(VmbdTuning)]),
     'vmbd_plan_workspace_bytes': (ctypes.c_size_t, [_P]),
     'vmbd_plan_out_shape': (ctypes.c_int, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
     'vmbd_plan_spectrum_elems': (ctypes.c_size_t, [_P]),

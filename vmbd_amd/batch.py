@@ -24,7 +24,9 @@ launch forms:
   another's.
 
 ``capture()`` records one step in a HIP graph and returns its replay.  With
-``tune=True`` each group's FFT tile comes from the autotuner.
+``tune=True`` each group's FFT tile and its plan's launch shapes come from
+the autotuner (the grouped launches take the tile; their launch shapes are
+chosen per launch).
 """
 import ctypes
 from collections import OrderedDict
@@ -53,11 +55,12 @@ class MixedBatch:
         self.where = {}
         for (sx, csf, sf), idx in groups.items():
             wa = win2winaux(sx, csf, overlap, device=self.device)
-            tile = None
+            cfg = dict(tile=None, ola_threads=0, col_threads=0)
             if tune:
-                from .autotune import best_tile
-                tile, _ = best_tile(sx, sf, wa, frames=len(idx), overlap=overlap, device=self.device)
-            plan = OlaPlan(sx, sf, wa, 'valid', tile=tile, device=self.device, max_frames=len(idx))
+                from .autotune import best_config
+                cfg, _ = best_config(sx, sf, wa, frames=len(idx), overlap=overlap, device=self.device)
+            plan = OlaPlan(sx, sf, wa, 'valid', tile=cfg['tile'], device=self.device, max_frames=len(idx))
+            plan.set_tuning(cfg['ola_threads'], cfg['col_threads'])
             n = len(idx)
             d, f32 = self.device, torch.float32
             bufs = dict(x=torch.zeros((n,) + sx, dtype=f32, device=d),

@@ -37,6 +37,7 @@ struct vmbd_plan {
   int mc_fwd = 0, mc_adj = 0;
   int max_frames = 1;
   int rows_frames = 0;  // >0: wsX/wsF hold the row transforms of the last forward (that many frames)
+  vmbd_tuning tune{};   // launch shapes (0 = automatic)
 };
 
 namespace {
@@ -423,6 +424,27 @@ vmbd_plan* vmbd_plan_create(const vmbd_geom* gm, const float* win_table, int32_t
 
 void vmbd_plan_destroy(vmbd_plan* plan) { plan_free(plan); }
 
+int vmbd_plan_set_tuning(vmbd_plan* p, const vmbd_tuning* t) {
+  vmbd_clear_error();
+  if (!p || !t) return vmbd_fail(VMBD_E_ARG, "vmbd_plan_set_tuning: null argument");
+  const int o = t->ola_threads, c = t->col_threads;
+  if (o != 0 && o != 128 && o != 192 && o != 256)
+    return vmbd_fail(VMBD_E_ARG, "vmbd_plan_set_tuning: ola_threads=%d not in {0, 128, 192, 256}", o);
+  if (c != 0 && c != 128 && c != 256)
+    return vmbd_fail(VMBD_E_ARG, "vmbd_plan_set_tuning: col_threads=%d not in {0, 128, 256}", c);
+  if ((o == 128 || o == 192) && !(ola_fits(p->g, dst_forward(p, nullptr), o) &&
+                                  ola_fits(p->g, dst_adjoint_x(p, nullptr), o)))
+    return vmbd_fail(VMBD_E_UNSUP, "vmbd_plan_set_tuning: an output row needs more than one round of %d threads", o);
+  p->tune = *t;
+  return VMBD_OK;
+}
+
+int vmbd_plan_get_tuning(const vmbd_plan* p, vmbd_tuning* t) {
+  if (!p || !t) return vmbd_fail(VMBD_E_ARG, "vmbd_plan_get_tuning: null argument");
+  *t = p->tune;
+  return VMBD_OK;
+}
+
 size_t vmbd_plan_workspace_bytes(const vmbd_plan* plan) { return plan ? plan->bytes : 0; }
 
 int vmbd_plan_out_shape(const vmbd_plan* plan, int32_t* sy0, int32_t* sy1) {
@@ -451,8 +473,8 @@ int vmbd_ola_forward(vmbd_plan* p, int32_t nframes, const float* x, const float*
   ColOut o1{p->Y1, g.ext0}, o2{nullptr, 0};
   ColIn none{nullptr, 0, 0, 0};
   VMBD_LAUNCH(launch_cols(g, COL_FWD, raw_in(p->wsX, srcs[0]), raw_in(p->wsF, srcs[1]), none, o1, o2, nullptr,
-                          p->tw, st));
-  VMBD_LAUNCH(launch_rows_c2r_ola(g, dst_forward(p, y), p->tw, st));
+                          p->tw, st, p->tune.col_threads));
+  VMBD_LAUNCH(launch_rows_c2r_ola(g, dst_forward(p, y), p->tw, st, p->tune.ola_threads));
   p->rows_frames = nframes;
   return VMBD_OK;
 }
@@ -471,8 +493,8 @@ int vmbd_ola_adjoint_x(vmbd_plan* p, int32_t nframes, const float* gres, const f
   VMBD_LAUNCH(launch_rows_r2c(g, sf, p->wsF, p->tw, st));
   ColOut o1{p->Y1, g.sw0}, o2{nullptr, 0};
   ColIn none{};
-  VMBD_LAUNCH(launch_cols(g, COL_ADJ1, shared_in(p->wsG, sr), raw_in(p->wsF, sf), none, o1, o2, nullptr, p->tw, st));
-  VMBD_LAUNCH(launch_rows_c2r_ola(g, dst_adjoint_x(p, xbar), p->tw, st));
+  VMBD_LAUNCH(launch_cols(g, COL_ADJ1, shared_in(p->wsG, sr), raw_in(p->wsF, sf), none, o1, o2, nullptr, p->tw, st, p->tune.col_threads));
+  VMBD_LAUNCH(launch_rows_c2r_ola(g, dst_adjoint_x(p, xbar), p->tw, st, p->tune.ola_threads));
   return VMBD_OK;
 }
 
@@ -490,7 +512,7 @@ int vmbd_ola_adjoint_f(vmbd_plan* p, int32_t nframes, const float* gres, const f
   VMBD_LAUNCH(launch_rows_r2c(g, sx, p->wsX, p->tw, st));
   ColOut o1{p->Y2, g.sf0}, o2{nullptr, 0};
   ColIn none{};
-  VMBD_LAUNCH(launch_cols(g, COL_ADJ1, shared_in(p->wsG, sr), raw_in(p->wsX, sx), none, o1, o2, nullptr, p->tw, st));
+  VMBD_LAUNCH(launch_cols(g, COL_ADJ1, shared_in(p->wsG, sr), raw_in(p->wsX, sx), none, o1, o2, nullptr, p->tw, st, p->tune.col_threads));
   VMBD_LAUNCH(launch_rows_c2r_stack(g, p->Y2, g.sf0, fbar, g.sf0, g.sf1, inv_scale(g), p->tw, st));
   return VMBD_OK;
 }
@@ -519,8 +541,8 @@ int vmbd_ola_adjoint_xf(vmbd_plan* p, int32_t nframes, const float* gres, const 
   }
   ColOut o1{p->Y1, g.sw0}, o2{p->Y2, g.sf0};
   VMBD_LAUNCH(launch_cols(g, COL_ADJ2, shared_in(p->wsG, sr), raw_in(p->wsF, sf), raw_in(p->wsX, sx), o1, o2, nullptr,
-                          p->tw, st));
-  VMBD_LAUNCH(launch_rows_c2r_ola(g, dst_adjoint_x(p, xbar), p->tw, st));
+                          p->tw, st, p->tune.col_threads));
+  VMBD_LAUNCH(launch_rows_c2r_ola(g, dst_adjoint_x(p, xbar), p->tw, st, p->tune.ola_threads));
   VMBD_LAUNCH(launch_rows_c2r_stack(g, p->Y2, g.sf0, fbar, g.sf0, g.sf1, inv_scale(g), p->tw, st));
   return VMBD_OK;
 }
@@ -570,7 +592,7 @@ int vmbd_spectra(vmbd_plan* p, int32_t nframes, int32_t kind, const float* z, vo
   VMBD_LAUNCH(launch_rows_r2c(g, s, ws, p->tw, st));
   ColOut none_o{nullptr, 0};
   ColIn none{nullptr, 0, 0, 0};
-  VMBD_LAUNCH(launch_cols(g, COL_SPEC, raw_in(ws, s), none, none, none_o, none_o, (float2*)spec, p->tw, st));
+  VMBD_LAUNCH(launch_cols(g, COL_SPEC, raw_in(ws, s), none, none, none_o, none_o, (float2*)spec, p->tw, st, p->tune.col_threads));
   return VMBD_OK;
 }
 
@@ -590,11 +612,11 @@ int vmbd_ola_forward_cached_ex(vmbd_plan* p, int32_t nframes, int32_t kind, cons
   VMBD_LAUNCH(launch_rows_r2c(g, s, ws, p->tw, st));
   ColOut o1{p->Y1, g.ext0}, o2{nullptr, 0};
   ColIn none{};
-  VMBD_LAUNCH(launch_cols(g, COL_FWD, raw_in(ws, s), spec_in(spec, g.Th), none, o1, o2, nullptr, p->tw, st));
+  VMBD_LAUNCH(launch_cols(g, COL_FWD, raw_in(ws, s), spec_in(spec, g.Th), none, o1, o2, nullptr, p->tw, st, p->tune.col_threads));
   OlaDst d = dst_forward(p, y);
   d.sub = sub;
   d.mul = mul;
-  VMBD_LAUNCH(launch_rows_c2r_ola(g, d, p->tw, st));
+  VMBD_LAUNCH(launch_rows_c2r_ola(g, d, p->tw, st, p->tune.ola_threads));
   return VMBD_OK;
 }
 
@@ -618,14 +640,14 @@ int vmbd_ola_adjoint_cached_ex(vmbd_plan* p, int32_t nframes, int32_t kind, cons
   ColOut o2{nullptr, 0};
   if (kind == VMBD_CACHE_PSF) {
     ColOut o1{p->Y1, g.sw0};
-    VMBD_LAUNCH(launch_cols(g, COL_ADJ1, shared_in(p->wsG, sr), spec_in(spec, g.Th), none, o1, o2, nullptr, p->tw, st));
+    VMBD_LAUNCH(launch_cols(g, COL_ADJ1, shared_in(p->wsG, sr), spec_in(spec, g.Th), none, o1, o2, nullptr, p->tw, st, p->tune.col_threads));
     OlaDst d = dst_adjoint_x(p, out);
     d.axz = z;
     d.beta = beta;
-    VMBD_LAUNCH(launch_rows_c2r_ola(g, d, p->tw, st));
+    VMBD_LAUNCH(launch_rows_c2r_ola(g, d, p->tw, st, p->tune.ola_threads));
   } else {
     ColOut o1{p->Y2, g.sf0};
-    VMBD_LAUNCH(launch_cols(g, COL_ADJ1, shared_in(p->wsG, sr), spec_in(spec, g.Th), none, o1, o2, nullptr, p->tw, st));
+    VMBD_LAUNCH(launch_cols(g, COL_ADJ1, shared_in(p->wsG, sr), spec_in(spec, g.Th), none, o1, o2, nullptr, p->tw, st, p->tune.col_threads));
     VMBD_LAUNCH(launch_rows_c2r_stack(g, p->Y2, g.sf0, out, g.sf0, g.sf1, inv_scale(g), p->tw, st, z, beta));
   }
   return VMBD_OK;
@@ -787,7 +809,7 @@ int vmbd_deconv_sparse_setup(vmbd_plan* p, const void* spec, const float* y, con
   OlaDst d = dst_patches(p, cfac, true);
   d.clip_lo = 1;
   d.lo = 0.01f;
-  VMBD_LAUNCH(launch_rows_c2r_ola(g, d, p->tw, st));
+  VMBD_LAUNCH(launch_rows_c2r_ola(g, d, p->tw, st, p->tune.ola_threads));
   // cfac = 0.01 / max(const, 0.01)
   return vmbd_ew(cfac, cfac, nullptr, 0.01f, (int64_t)g.H * g.W, VMBD_EW_SDIV, stream);
 }
@@ -817,7 +839,7 @@ int vmbd_deconv_sparse_step(vmbd_plan* p, const void* spec, const void* ftpy, co
   d.mul = cfac;
   d.clip_lo = 1;
   d.lo = 0.f;
-  VMBD_LAUNCH(launch_rows_c2r_ola(g, d, p->tw, st));
+  VMBD_LAUNCH(launch_rows_c2r_ola(g, d, p->tw, st, p->tune.ola_threads));
   return VMBD_OK;
 }
 

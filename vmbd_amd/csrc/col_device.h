@@ -46,12 +46,13 @@ __device__ __forceinline__ void col_load(float2 (&v)[16], const ColIn& in, int p
   }
 }
 
-// stage rows [0,nrows) of the strip in LDS and store row-major [P][nrows][NBP].
+// stage rows [0,nrows) of the strip in LDS and store row-major [P][nrows][NBP]
+// (WGC threads per workgroup).
 // The staging area aliases the FFT slices: barriers on both sides, except
 // after the kernel's last store (LAST), where no slice is used again.
 // MM < 16: the caller guarantees nrows <= MM*TPT, so slots m >= MM are never
 // stored (and the transform's outputs there are dead code).
-template <int TH, int CS, bool LAST = false, int MM = 16>
+template <int TH, int CS, bool LAST = false, int MM = 16, int WGC = WG>
 __device__ __forceinline__ void col_store_rows(const float2 (&v)[16], float2* st, const ColOut& o, int p, int strip,
                                                int NB, int NBP, int cc, int t) {
   constexpr int TPT = TH / 16;
@@ -66,7 +67,7 @@ __device__ __forceinline__ void col_store_rows(const float2 (&v)[16], float2* st
   const int ci = threadIdx.x & (CS - 1);
   if (c0 + ci < NB - 1) {  // bin NB-1 travels packed in bin 0
     float2* dst = o.p + (size_t)p * o.nrows * NBP + c0 + ci;
-    for (int r = threadIdx.x / CS; r < o.nrows; r += WG / CS) dst[(size_t)r * NBP] = st[r * (CS + 1) + ci];
+    for (int r = threadIdx.x / CS; r < o.nrows; r += WGC / CS) dst[(size_t)r * NBP] = st[r * (CS + 1) + ci];
   }
   if (!LAST) __syncthreads();  // staging drained before the slices are reused
 }

@@ -139,9 +139,15 @@ hipError_t launch_rows_r2c(const Geo& g, const RowSrc& s, float2* out, const flo
 // several row-transform jobs in one launch (n <= ROW_JOBS_MAX)
 hipError_t launch_rows_r2c_multi(const Geo& g, const RowSrc* s, float2* const* out, int n, const float2* tw,
                                  hipStream_t st);
+// wgc: threads per column-pass workgroup (128 or 256; 0 = 256): the column
+// strip of a workgroup is wgc / (Th/16) columns wide
 hipError_t launch_cols(const Geo& g, int op, const ColIn& a, const ColIn& b, const ColIn& c, const ColOut& o1,
-                       const ColOut& o2, float2* spec_out, const float2* tw, hipStream_t st);
-hipError_t launch_rows_c2r_ola(const Geo& g, const OlaDst& d, const float2* tw, hipStream_t st);
+                       const ColOut& o2, float2* spec_out, const float2* tw, hipStream_t st, int wgc = 0);
+// wgs: threads per OLA workgroup (128, 192 or 256; 0 = ola_wg's choice);
+// 128 and 192 need every output row in one round (ola_fits)
+hipError_t launch_rows_c2r_ola(const Geo& g, const OlaDst& d, const float2* tw, hipStream_t st, int wgs = 0);
+// a single-round launch of wgs threads holds every output row's transforms
+int ola_fits(const Geo& g, const OlaDst& d, int wgs);
 // Column pass of a Fourier division (deconv_kernels.hip): operands a[0..na)
 // (row-pass outputs or spectra, in the order of DivOp's formula: DIV_X /
 // SPROD: A, B; DIV_G: B1, A1, B2, A2; DIV_S: A1, A2, S1, S2); result rows

@@ -353,22 +353,22 @@ hipError_t launch_rows_r2c(const Geo& g, const RowSrc& s, float2* out, const flo
 
 // LDS (float2) of one column-pass workgroup: the transforms' slices, aliased
 // by the row staging of the store
-template <int TH, int OP>
+template <int TH, int OP, int WGC = WG>
 struct ColsLds {
-  static constexpr int CS = WG / (TH / 16);
+  static constexpr int CS = WGC / (TH / 16);
   static constexpr int v = (OP == COL_SPEC) ? CS * FftLds<TH>::floats2 : cmax<CS * FftLds<TH>::floats2, TH * (CS + 1)>::v;
 };
 
-// One workgroup of the column pass: strip `strip` of patch p (sm:
-// ColsLds<TH, OP>::v float2 of LDS).
-template <int TH, int OP>
+// One workgroup of WGC threads of the column pass: strip `strip` of patch p
+// (sm: ColsLds<TH, OP, WGC>::v float2 of LDS).
+template <int TH, int OP, int WGC = WG>
 // (parameters by value: with references the compiler keeps the column
 // descriptors in vector registers and spills at 256 points)
 __device__ __forceinline__ void cols_body(Geo g, ColIn a, ColIn b, ColIn c3, ColOut o1, ColOut o2,
                                           float2* __restrict__ spec_out, int strip, int p, float2* sm,
                                           const float2* __restrict__ tw) {
   constexpr int TPT = TH / 16;
-  constexpr int CS = WG / TPT;  // columns per workgroup
+  constexpr int CS = WGC / TPT;  // columns per workgroup
   constexpr int SL = FftLds<TH>::floats2;
 
   const int NB = g.NB;
@@ -438,10 +438,10 @@ __device__ __forceinline__ void cols_body(Geo g, ColIn a, ColIn b, ColIn c3, Col
     }
     if (OP == COL_ADJ1 && o1.nrows <= 3 * TPT) {
       if (wact) fft<TH, +1>(B, sl, t, T, tw);
-      col_store_rows<TH, CS, true, 3>(B, sm, o1, p, strip, NB, g.NBP, cc, t);
+      col_store_rows<TH, CS, true, 3, WGC>(B, sm, o1, p, strip, NB, g.NBP, cc, t);
     } else {
       if (wact) fft<TH, +1>(B, sl, t, T, tw);
-      col_store_rows<TH, CS, OP != COL_ADJ2>(B, sm, o1, p, strip, NB, g.NBP, cc, t);
+      col_store_rows<TH, CS, OP != COL_ADJ2, 16, WGC>(B, sm, o1, p, strip, NB, g.NBP, cc, t);
     }
     if constexpr (OP == COL_ADJ2) {
       col_load<TH>(B, c3, p, c, NB, t, active, grp, row0);
@@ -452,9 +452,9 @@ __device__ __forceinline__ void cols_body(Geo g, ColIn a, ColIn b, ColIn c3, Col
       }
       if (wact) fft<TH, +1>(B, sl, t, T, tw);
       if (PRUNE && o2.nrows <= 3 * TPT)
-        col_store_rows<TH, CS, true, 3>(B, sm, o2, p, strip, NB, g.NBP, cc, t);
+        col_store_rows<TH, CS, true, 3, WGC>(B, sm, o2, p, strip, NB, g.NBP, cc, t);
       else
-        col_store_rows<TH, CS, true>(B, sm, o2, p, strip, NB, g.NBP, cc, t);
+        col_store_rows<TH, CS, true, 16, WGC>(B, sm, o2, p, strip, NB, g.NBP, cc, t);
     }
   }
 }
@@ -464,36 +464,39 @@ __device__ __forceinline__ void cols_body(Geo g, ColIn a, ColIn b, ColIn c3, Col
 // workgroups per CU.  Strips of one patch are consecutive workgroups; each
 // writes whole 128-byte lines of its rows (16 bins, row pitch NBP a multiple
 // of 16).
-template <int TH, int OP>
-__global__ __launch_bounds__(WG, (TH >= 256 ? 4 : 1)) void k_cols(Geo g, ColIn a, ColIn b, ColIn c3, ColOut o1, ColOut o2,
-                                             float2* __restrict__ spec_out, const float2* __restrict__ tw) {
-  __shared__ float2 sm[ColsLds<TH, OP>::v];
+template <int TH, int OP, int WGC>
+__global__ __launch_bounds__(WGC, (TH >= 256 ? 4 * WG / WGC : 1)) void k_cols(Geo g, ColIn a, ColIn b, ColIn c3,
+                                                                             ColOut o1, ColOut o2,
+                                                                             float2* __restrict__ spec_out,
+                                                                             const float2* __restrict__ tw) {
+  __shared__ float2 sm[ColsLds<TH, OP, WGC>::v];
   // frame blockIdx.z, patch blockIdx.y
-  cols_body<TH, OP>(g, a, b, c3, o1, o2, spec_out, blockIdx.x, (int)blockIdx.z * g.Pf + (int)blockIdx.y, sm, tw);
+  cols_body<TH, OP, WGC>(g, a, b, c3, o1, o2, spec_out, blockIdx.x, (int)blockIdx.z * g.Pf + (int)blockIdx.y, sm,
+                         tw);
 }
 
 #undef VMBD_COL_PRODUCT
 
-template <int TH>
-static hipError_t cols_t(const Geo& g, int op, const ColIn& a, const ColIn& b, const ColIn& c, const ColOut& o1,
-                         const ColOut& o2, float2* spec_out, const float2* tw, hipStream_t st) {
-  constexpr int CS = WG / (TH / 16);
+template <int TH, int WGC>
+static hipError_t cols_wg(const Geo& g, int op, const ColIn& a, const ColIn& b, const ColIn& c, const ColOut& o1,
+                          const ColOut& o2, float2* spec_out, const float2* tw, hipStream_t st) {
+  constexpr int CS = WGC / (TH / 16);
   const int strips = (g.NB - 1 + CS - 1) / CS;  // bins [0, NB-1): NB-1 is packed into bin 0
   const dim3 grid((unsigned)strips, (unsigned)g.Pf, (unsigned)(g.P / g.Pf));
   static const char* names[4] = {"cols_spec", "cols_fwd", "cols_adj1", "cols_adj2"};
   ProfScope ps(names[op & 3], st);
   switch (op) {
     case COL_SPEC:
-      hipLaunchKernelGGL((k_cols<TH, COL_SPEC>), grid, dim3(WG), 0, st, g, a, b, c, o1, o2, spec_out, tw);
+      hipLaunchKernelGGL((k_cols<TH, COL_SPEC, WGC>), grid, dim3(WGC), 0, st, g, a, b, c, o1, o2, spec_out, tw);
       break;
     case COL_FWD:
-      hipLaunchKernelGGL((k_cols<TH, COL_FWD>), grid, dim3(WG), 0, st, g, a, b, c, o1, o2, spec_out, tw);
+      hipLaunchKernelGGL((k_cols<TH, COL_FWD, WGC>), grid, dim3(WGC), 0, st, g, a, b, c, o1, o2, spec_out, tw);
       break;
     case COL_ADJ1:
-      hipLaunchKernelGGL((k_cols<TH, COL_ADJ1>), grid, dim3(WG), 0, st, g, a, b, c, o1, o2, spec_out, tw);
+      hipLaunchKernelGGL((k_cols<TH, COL_ADJ1, WGC>), grid, dim3(WGC), 0, st, g, a, b, c, o1, o2, spec_out, tw);
       break;
     case COL_ADJ2:
-      hipLaunchKernelGGL((k_cols<TH, COL_ADJ2>), grid, dim3(WG), 0, st, g, a, b, c, o1, o2, spec_out, tw);
+      hipLaunchKernelGGL((k_cols<TH, COL_ADJ2, WGC>), grid, dim3(WGC), 0, st, g, a, b, c, o1, o2, spec_out, tw);
       break;
     default:
       return hipErrorInvalidValue;
@@ -501,15 +504,23 @@ static hipError_t cols_t(const Geo& g, int op, const ColIn& a, const ColIn& b, c
   return hipGetLastError();
 }
 
+template <int TH>
+static hipError_t cols_t(const Geo& g, int op, const ColIn& a, const ColIn& b, const ColIn& c, const ColOut& o1,
+                         const ColOut& o2, float2* spec_out, const float2* tw, hipStream_t st, int wgc) {
+  if (wgc == 128) return cols_wg<TH, 128>(g, op, a, b, c, o1, o2, spec_out, tw, st);
+  return cols_wg<TH, WG>(g, op, a, b, c, o1, o2, spec_out, tw, st);
+}
+
 hipError_t launch_cols(const Geo& g, int op, const ColIn& a, const ColIn& b, const ColIn& c, const ColOut& o1,
-                       const ColOut& o2, float2* spec_out, const float2* tw, hipStream_t st) {
+                       const ColOut& o2, float2* spec_out, const float2* tw, hipStream_t st, int wgc) {
+  if (wgc != 0 && wgc != 128 && wgc != WG) return hipErrorInvalidValue;
   switch (g.Th) {
-    case 16: return cols_t<16>(g, op, a, b, c, o1, o2, spec_out, tw, st);
-    case 32: return cols_t<32>(g, op, a, b, c, o1, o2, spec_out, tw, st);
-    case 64: return cols_t<64>(g, op, a, b, c, o1, o2, spec_out, tw, st);
-    case 128: return cols_t<128>(g, op, a, b, c, o1, o2, spec_out, tw, st);
-    case 256: return cols_t<256>(g, op, a, b, c, o1, o2, spec_out, tw, st);
-    case 512: return cols_t<512>(g, op, a, b, c, o1, o2, spec_out, tw, st);
+    case 16: return cols_t<16>(g, op, a, b, c, o1, o2, spec_out, tw, st, wgc);
+    case 32: return cols_t<32>(g, op, a, b, c, o1, o2, spec_out, tw, st, wgc);
+    case 64: return cols_t<64>(g, op, a, b, c, o1, o2, spec_out, tw, st, wgc);
+    case 128: return cols_t<128>(g, op, a, b, c, o1, o2, spec_out, tw, st, wgc);
+    case 256: return cols_t<256>(g, op, a, b, c, o1, o2, spec_out, tw, st, wgc);
+    case 512: return cols_t<512>(g, op, a, b, c, o1, o2, spec_out, tw, st, wgc);
     default: return hipErrorInvalidValue;
   }
 }
@@ -895,13 +906,19 @@ static void c2r_ola_launch(const Geo& g, const OlaDst& d, const float2* tw, hipS
     hipLaunchKernelGGL((k_rows_c2r_ola<TW, false, false, WGS>), grid, dim3(WGS), shm, st, g, d, tw);
 }
 
+int ola_fits(const Geo& g, const OlaDst& d, int wgs) {
+  return d.alias && (g.Tw / 16) * d.mc * ((g.csf1 + 1) / 2) <= wgs;
+}
+
 template <int TW>
-static hipError_t c2r_ola_t(const Geo& g, const OlaDst& d, const float2* tw, hipStream_t st) {
+static hipError_t c2r_ola_t(const Geo& g, const OlaDst& d, const float2* tw, hipStream_t st, int wgs) {
   const int frames = g.P / g.Pf;
   const size_t shm = ola_shared_bytes(g, d);
   ProfScope ps(d.win ? "rows_c2r_ola_win" : "rows_c2r_ola", st);
   const dim3 grid((unsigned)d.O0, (unsigned)frames);
-  switch (ola_wg(g, d, (long long)d.O0 * frames)) {
+  if (wgs == 0) wgs = ola_wg(g, d, (long long)d.O0 * frames);
+  if (wgs != WG && !ola_fits(g, d, wgs)) return hipErrorInvalidValue;
+  switch (wgs) {
     case 128: c2r_ola_launch<TW, 128>(g, d, tw, st, grid, shm); break;
     case 192: c2r_ola_launch<TW, 192>(g, d, tw, st, grid, shm); break;
     default: c2r_ola_launch<TW, WG>(g, d, tw, st, grid, shm); break;
@@ -909,16 +926,17 @@ static hipError_t c2r_ola_t(const Geo& g, const OlaDst& d, const float2* tw, hip
   return hipGetLastError();
 }
 
-hipError_t launch_rows_c2r_ola(const Geo& g, const OlaDst& d, const float2* tw, hipStream_t st) {
+hipError_t launch_rows_c2r_ola(const Geo& g, const OlaDst& d, const float2* tw, hipStream_t st, int wgs) {
   if (d.O0 <= 0 || d.O1 <= 0) return hipSuccess;
   if (g.NBP < g.Tw / 2 + 2) return hipErrorInvalidValue;  // two-bin loads need a padding bin per row
+  if (wgs != 0 && wgs != 128 && wgs != 192 && wgs != WG) return hipErrorInvalidValue;
   switch (g.Tw) {
-    case 16: return c2r_ola_t<16>(g, d, tw, st);
-    case 32: return c2r_ola_t<32>(g, d, tw, st);
-    case 64: return c2r_ola_t<64>(g, d, tw, st);
-    case 128: return c2r_ola_t<128>(g, d, tw, st);
-    case 256: return c2r_ola_t<256>(g, d, tw, st);
-    case 512: return c2r_ola_t<512>(g, d, tw, st);
+    case 16: return c2r_ola_t<16>(g, d, tw, st, wgs);
+    case 32: return c2r_ola_t<32>(g, d, tw, st, wgs);
+    case 64: return c2r_ola_t<64>(g, d, tw, st, wgs);
+    case 128: return c2r_ola_t<128>(g, d, tw, st, wgs);
+    case 256: return c2r_ola_t<256>(g, d, tw, st, wgs);
+    case 512: return c2r_ola_t<512>(g, d, tw, st, wgs);
     default: return hipErrorInvalidValue;
   }
 }

@@ -70,7 +70,7 @@ class OnlineMFBD:
 
     def __init__(self, y0, sf, csf=(3, 3), overlap=0.5, f_solver='lbfgsb', x_solver='rl', f_alpha=0.,
                  f_beta=0.1, optiter=500, tol=1e-10, f_iters=20, x_iters=20, exact=False, seed=120, capture=False,
-                 f_init='warm', prior=None, device=None):
+                 f_init='warm', prior=None, device=None, tune=False):
         if f_solver not in ('lbfgsb', 'pbb'):
             raise ValueError("f_solver must be 'lbfgsb' or 'pbb'")
         if x_solver not in ('rl', 'pbb'):
@@ -101,7 +101,15 @@ class OnlineMFBD:
         self.capture = bool(capture)
         self.gen = torch.Generator(device=self.device)
         self.gen.manual_seed(int(seed))
-        self.plan = OlaPlan(self.sx, self.sf, self.winaux, 'valid', device=self.device)
+        # the FFT tile and launch shapes of the plan: the smallest tile and the
+        # library's own shapes, or (tune=True) the autotuner's fastest
+        self.tuning = dict(tile=None, ola_threads=0, col_threads=0)
+        if tune:
+            from .autotune import best_config
+            self.tuning, _ = best_config(self.sx, self.sf, self.winaux, frames=1, overlap=overlap,
+                                         device=self.device, workload='cached')
+        self.plan = OlaPlan(self.sx, self.sf, self.winaux, 'valid', tile=self.tuning['tile'], device=self.device)
+        self.plan.set_tuning(self.tuning['ola_threads'], self.tuning['col_threads'])
         # vmbd_chimney.py:97: x = impad(y0, sf - 1)
         self.x = gputools.impad_gpu(y0, (self.sf[0] - 1, self.sf[1] - 1))
         self.f = None

@@ -76,6 +76,20 @@ class OlaPlan:
         self.workspace_bytes = int(L.vmbd_plan_workspace_bytes(self._h))
         self._rows_key = None
 
+    # -- launch shapes --------------------------------------------------------
+    def set_tuning(self, ola_threads=0, col_threads=0):
+        """Threads per workgroup of the overlap-add passes (128 / 192 / 256)
+        and of the column passes (128 / 256); 0 = automatic.  Results are
+        the same bits for every setting."""
+        t = _lib.VmbdTuning(int(ola_threads), int(col_threads))
+        _lib.check(_lib.lib().vmbd_plan_set_tuning(self._h, ctypes.byref(t)), 'set_tuning')
+
+    @property
+    def tuning(self):
+        t = _lib.VmbdTuning()
+        _lib.check(_lib.lib().vmbd_plan_get_tuning(self._h, ctypes.byref(t)), 'get_tuning')
+        return dict(ola_threads=t.ola_threads, col_threads=t.col_threads)
+
     def __del__(self):
         h = getattr(self, '_h', None)
         if h:
