@@ -205,8 +205,7 @@ OlaDst dst_forward(const vmbd_plan* p, float* y) {
   d.win = nullptr;
   d.win_idx = nullptr;
   d.mc = p->mc_fwd;
-  d.alias = ola_single_round(g, d);
-  d.stream = ola_stream_rounds(g, d);
+  ola_layout(g, d);
   return d;
 }
 
@@ -229,8 +228,7 @@ OlaDst dst_adjoint_x(const vmbd_plan* p, float* xbar) {
   d.row_class = p->row_class;
   d.nslot = p->nslot;
   d.mc = p->mc_adj;
-  d.alias = ola_single_round(g, d);
-  d.stream = ola_stream_rounds(g, d);
+  ola_layout(g, d);
   return d;
 }
 

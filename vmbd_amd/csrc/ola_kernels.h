@@ -103,6 +103,11 @@ struct OlaDst {
   int alias;   // 1: all transforms of a row fit one round; gather buffer aliases the FFT slices
   int stream;  // 1: several rounds, each round's rows (aliasing the FFT slices) gathered into
                //    per-thread column accumulators; needs O1 <= ola_stream_cols()
+  // > 0 (single-round launches): the gather buffer holds, per covering patch
+  // row, ncls rows of ola_class_len() floats in overlap-add coordinates;
+  // patch n writes its row at column n*hop1 of row n mod ncls, so one
+  // output column reads the same column of every row (ola_layout)
+  int ncls;
   // 1/(Th*Tw) of the unnormalised transforms, applied to the overlap-add sum
   float scale;
   // fused solver arithmetic on the stored value v = scale * sum (each optional):
@@ -201,6 +206,13 @@ hipError_t launch_batch(const BatchGroup* dev_groups, const BatchLaunch& L, int 
                         hipStream_t st);
 
 size_t ola_rowbuf_bytes(const Geo& g, const OlaDst& d);
+// floats per class row of the gather buffer: the overlap-add extent of a
+// patch row, (csf1-1)*hop1 + ext1, rounded up to 4
+__host__ __device__ inline int ola_class_len(const Geo& g, const OlaDst& d) {
+  return ((g.csf1 - 1) * g.hop1 + d.ext1 + 3) & ~3;
+}
+// sets d.alias, d.stream and d.ncls of a destination whose other fields are set
+void ola_layout(const Geo& g, OlaDst& d);
 size_t ola_shared_bytes(const Geo& g, const OlaDst& d);
 int ola_single_round(const Geo& g, const OlaDst& d);
 // multi-round launches whose output rows fit the per-thread accumulators

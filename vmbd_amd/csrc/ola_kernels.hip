@@ -544,16 +544,25 @@ struct C2r {
   static_assert(NI == 4, "chunk count per thread");
 };
 
+// Bins of an absent row: a transform without one of its rows reads these
+// zeros, so every lane issues the same loads without a per-lane test.
+__device__ const float4 c2r_zero_bins[512 / 4] = {};
+
 // global loads of both rows' chunks (all issued before any is used)
 template <int TW>
 __device__ __forceinline__ void c2r_fetch(float4 (&A)[4], float4 (&B)[4], const float2* ra, const float2* rb,
                                           bool hasa, bool hasb, int t) {
+  static_assert(C2r<TW>::NCH <= 512 / 4, "zero row shorter than a row's chunks");
+  const float2* pa = hasa ? ra : (const float2*)c2r_zero_bins;
+  const float2* pb = hasb ? rb : (const float2*)c2r_zero_bins;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const int k = t + C2r<TW>::TPT * i;
     A[i] = B[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (hasa && k < C2r<TW>::NCH) __builtin_memcpy(&A[i], ra + 2 * k, 16);
-    if (hasb && k < C2r<TW>::NCH) __builtin_memcpy(&B[i], rb + 2 * k, 16);
+    if (k < C2r<TW>::NCH) {
+      __builtin_memcpy(&A[i], pa + 2 * k, 16);
+      __builtin_memcpy(&B[i], pb + 2 * k, 16);
+    }
   }
 }
 
@@ -691,6 +700,9 @@ __device__ __forceinline__ void c2r_ola_body(Geo g, OlaDst d, int i, int fr, flo
   }
   const int Mc = mhi >= mlo ? (mhi - mlo + 1) : 0;
   const int NP2 = (g.csf1 + 1) >> 1;
+  // class layout of the gather buffer (single-round launches)
+  const int NC = STREAM ? 0 : d.ncls;
+  const int RLc = ola_class_len(g, d);
   const int NT = Mc * NP2;
   const int tr = threadIdx.x / TPT;
   const int t = threadIdx.x - tr * TPT;
@@ -739,9 +751,29 @@ __device__ __forceinline__ void c2r_ola_body(Geo g, OlaDst d, int i, int fr, flo
     }
     if (WIN && base == 0) __builtin_amdgcn_s_waitcnt(0);  // this wavefront's window loads have landed
     if (STREAM || d.alias || (WIN && base == 0)) __syncthreads();  // slices about to become rowbuf; windows staged
+    if (!STREAM && NC) {
+      // class rows start as zeros: the patches of a class leave gaps
+      float4* z4 = (float4*)rowbuf;
+      const int n4 = (Mc * NC * RLc) >> 2;
+      for (int e = threadIdx.x; e < n4; e += wgs) z4[e] = make_float4(0.f, 0.f, 0.f, 0.f);
+      __syncthreads();
+    }
     if (act) {
-      float* ba = rowbuf + (STREAM ? 2 * tr : mi * g.csf1 + na) * d.ext1;
-      float* bb = rowbuf + (STREAM ? 2 * tr + 1 : mi * g.csf1 + nb) * d.ext1;
+      float* ba;
+      float* bb;
+      if (STREAM) {
+        ba = rowbuf + 2 * tr * d.ext1;
+        bb = ba + d.ext1;
+      } else if (NC) {
+        // patch n's row at column n*hop1 of class row n mod NC
+        const int qa = na % NC;
+        const int qb = qa + 1 == NC ? 0 : qa + 1;
+        ba = rowbuf + (mi * NC + qa) * RLc + na * g.hop1;
+        bb = rowbuf + (mi * NC + qb) * RLc + nb * g.hop1;
+      } else {
+        ba = rowbuf + (mi * g.csf1 + na) * d.ext1;
+        bb = ba + d.ext1;
+      }
       // staged window values of this thread's columns (x-adjoint: ext1 == sw1)
       const float* wa = wl;
       const float* wbp = wl;
@@ -812,10 +844,25 @@ __device__ __forceinline__ void c2r_ola_body(Geo g, OlaDst d, int i, int fr, flo
   }
   __syncthreads();
   // fixed-order gather over covering patches
+  const int K = Mc * NC;
   for (int j = threadIdx.x; j < d.O1; j += wgs) {
     const int vcol = j + d.oc1;
     float acc = 0.f;
-    if (vcol >= 0) {
+    if (NC) {
+      // column vcol of every class row, in (patch row, class) order
+      if (vcol >= 0 && vcol < RLc) {
+        const float* cb = rowbuf + vcol;
+        int k = 0;
+        for (; k + 4 <= K; k += 4) {
+          const float a0 = cb[k * RLc], a1 = cb[(k + 1) * RLc], a2 = cb[(k + 2) * RLc], a3 = cb[(k + 3) * RLc];
+          acc += a0;
+          acc += a1;
+          acc += a2;
+          acc += a3;
+        }
+        for (; k < K; ++k) acc += cb[k * RLc];
+      }
+    } else if (vcol >= 0) {
       // nhi = min(csf1-1, floor(vcol/hop1)) via a float reciprocal + exact fix-up
       int qh = (int)((float)vcol * inv_hop1);
       qh += ((qh + 1) * g.hop1 <= vcol) - (qh * g.hop1 > vcol);
@@ -853,13 +900,14 @@ __host__ __device__ int ola_window_offset_floats(const Geo& g, const OlaDst& d) 
   const int TPW = WG / TPT;  // launches with fewer threads are single-round: TPW unused
   const int nsl = d.alias ? d.mc * ((g.csf1 + 1) / 2) : TPW;  // FFT slices in use
   const int fl = nsl * (g.Tw + g.Tw / 16) * 2;
-  const int rb = d.stream ? 2 * TPW * d.ext1 : d.mc * g.csf1 * d.ext1;
+  const int rb = d.stream ? 2 * TPW * d.ext1 : d.ncls ? d.mc * d.ncls * ola_class_len(g, d) : d.mc * g.csf1 * d.ext1;
   const int off = (d.alias || d.stream) ? (fl > rb ? fl : rb) : fl + rb;
   return (off + 3) & ~3;
 }
 
 size_t ola_rowbuf_bytes(const Geo& g, const OlaDst& d) {
   const int TPW = WG / (g.Tw / 16);
+  if (!d.stream && d.ncls) return (size_t)d.mc * d.ncls * ola_class_len(g, d) * sizeof(float);
   return (size_t)(d.stream ? 2 * TPW : d.mc * g.csf1) * d.ext1 * sizeof(float);
 }
 
@@ -876,6 +924,32 @@ int ola_single_round(const Geo& g, const OlaDst& d) {
 }
 
 int ola_stream_rounds(const Geo& g, const OlaDst& d) { return !ola_single_round(g, d) && d.O1 <= OLA_KJ * WG; }
+
+// Single-round launches gather from class rows (ncls = ceil(ext1/hop1)
+// classes, at most csf1; patches n and n + ncls of a class do not overlap):
+// an output column then reads one column of each class row with fixed
+// strides instead of searching the covering patches.  The buffer is larger
+// than one row per patch when ncls > 2, so the layout is taken only if it
+// keeps the resident wavefronts per CU for every workgroup size the launch
+// may use.  Those are bounded by the LDS and by the kernels' registers
+// (about 104-112 VGPRs at Tw <= 256, four wavefronts per SIMD; three at 512).
+void ola_layout(const Geo& g, OlaDst& d) {
+  d.alias = ola_single_round(g, d);
+  d.stream = ola_stream_rounds(g, d);
+  d.ncls = 0;
+  if (!d.alias || g.hop1 <= 0) return;
+  OlaDst e = d;
+  e.ncls = std::min(g.csf1, (d.ext1 + g.hop1 - 1) / g.hop1);
+  const size_t b0 = ola_shared_bytes(g, d), b1 = ola_shared_bytes(g, e);
+  const int cap = g.Tw >= 512 ? 12 : 16;  // wavefronts per CU the registers allow
+  auto waves = [cap](size_t b, int wpg) {
+    const int by_lds = b == 0 ? 64 : (int)((size_t)160 * 1024 / b);
+    return std::min(cap / wpg, by_lds) * wpg;
+  };
+  for (int wgs = 128; wgs <= WG; wgs += 64)
+    if ((wgs == WG || ola_fits(g, d, wgs)) && waves(b1, wgs / 64) < waves(b0, wgs / 64)) return;
+  d.ncls = e.ncls;
+}
 
 // threads per workgroup: a single-round launch whose grid fills the GPU
 // many times over takes 128, 192 or 256 threads, the first that covers its
