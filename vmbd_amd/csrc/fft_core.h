@@ -211,6 +211,42 @@ __device__ __forceinline__ void twiddles_init(Twiddles<N>& T, int t, const float
   }
 }
 
+// The same twiddles in two steps: twiddles_seed issues the table loads (two
+// per butterfly), twiddles_from_seed forms the powers, so the loads can be in
+// flight with other work while only the seeds hold registers.
+template <int N>
+struct TwSeed {
+  static constexpr int NB1 = Twiddles<N>::NB1;
+  float2 w1[NB1], w4[NB1];
+};
+template <int N>
+__device__ __forceinline__ void twiddles_seed(TwSeed<N>& S, int t, const float2* __restrict__ tw) {
+  constexpr int TPT = N / 16;
+  constexpr int R1 = Plan<N>::R1;
+  if constexpr (R1 > 1) {
+    constexpr int STR = TW_LEN / (16 * R1);
+#pragma unroll
+    for (int b = 0; b < TwSeed<N>::NB1; ++b) {
+      const int k = (t + b * TPT) & 15;
+      S.w1[b] = tw[k * STR];
+      S.w4[b] = tw[(4 * k * STR) & (TW_LEN - 1)];
+    }
+  }
+}
+template <int N>
+__device__ __forceinline__ void twiddles_from_seed(Twiddles<N>& T, const TwSeed<N>& S) {
+  constexpr int R1 = Plan<N>::R1;
+  if constexpr (R1 > 1) {
+#pragma unroll
+    for (int b = 0; b < TwSeed<N>::NB1; ++b) {
+      float2 p[R1];
+      twiddle_powers<R1>(S.w1[b], S.w4[b], p);
+#pragma unroll
+      for (int r = 1; r < R1; ++r) T.w[b * (R1 - 1) + r - 1] = p[r];
+    }
+  }
+}
+
 // One Stockham pass after the first (Ns > 1).  Reads 16 values of the slice,
 // applies 16/R butterflies, writes 16 values (or keeps them in registers when
 // LAST: slot m = b + r*(16/R) holds X[t + TPT*m]).  Twiddles come from `pre`

@@ -108,6 +108,10 @@ struct OlaDst {
   // patch n writes its row at column n*hop1 of row n mod ncls, so one
   // output column reads the same column of every row (ola_layout)
   int ncls;
+  // > 0 (no window, class layout): the covering patch rows are summed in
+  // the row-frequency domain before one inverse transform per patch-column
+  // pair, sumr output rows per 256-thread workgroup (ola_layout)
+  int sumr;
   // 1/(Th*Tw) of the unnormalised transforms, applied to the overlap-add sum
   float scale;
   // fused solver arithmetic on the stored value v = scale * sum (each optional):

@@ -892,6 +892,148 @@ __global__ __launch_bounds__(WGS, (TW >= 512 ? 3 : 1)) void k_rows_c2r_ola(Geo g
   c2r_ola_body<TW, STREAM, WIN, WGS>(g, d, blockIdx.x, blockIdx.y, dyn, tw);
 }
 
+// Summed form of the overlap-add pass (no window, class layout): the inverse
+// row transform is linear and the patches (m, n) of one patch column n land
+// on the same output columns n*hop1 + c, so output row u is
+//   sum_n IFFT_row( sum_m Y[m, n][u - m*hop0] )  at columns n*hop1 + c:
+// one transform per (output row, pair of patch columns) instead of one per
+// (covering patch row, pair), 2.4x fewer at 50 % overlap.  R = d.sumr output
+// rows per workgroup share its TPW transforms (R * NP2 <= TPW).  A transform
+// fetches its covering rows two patch rows at a time (absent rows read the
+// zero row) and adds them in ascending patch-row order; its two patches' rows
+// go to their class rows (patch n to row n mod ncls of its output row), and
+// each output column adds its ncls class values.  Output rows i0 ..
+// i0 + R - 1 of frame fr (dyn: ola_shared_bytes of LDS).
+template <int TW>
+__device__ __forceinline__ void c2r_sum_body(Geo g, OlaDst d, int i0, int fr, float2* dyn,
+                                             const float2* __restrict__ tw) {
+  constexpr int TPT = TW / 16;
+  constexpr int SL = FftLds<TW>::floats2;
+  const int R = d.sumr;
+  const int NP2 = (g.csf1 + 1) >> 1;
+  const int NC = d.ncls;
+  const int RLc = ola_class_len(g, d);
+  float* rowbuf = (float*)dyn;  // class rows alias the FFT slices
+  const int tr = threadIdx.x / TPT;
+  const int t = threadIdx.x - tr * TPT;
+  const int ri = tr / NP2;
+  const int jp = tr - ri * NP2;
+  const int i = i0 + ri;
+  const bool act = ri < R && i < d.O0;
+  const int u = i + d.oc0;
+  // covering patch rows m: m*hop0 <= u < m*hop0 + ext0
+  int mlo = 0, mhi = -1;
+  if (act && u >= 0) {
+    mhi = min(g.csf0 - 1, u / g.hop0);
+    const int lo = u - d.ext0 + 1;
+    mlo = lo <= 0 ? 0 : (lo + g.hop0 - 1) / g.hop0;
+  }
+  const int na = 2 * jp, nb = na + 1;
+  const bool hasb = nb < g.csf1;
+  const int kf = fr * g.Pf;
+  float2* sl = dyn + tr * SL;
+  // a wavefront whose transforms all lie past the workgroup's rows skips them
+  const int trw = (int)(threadIdx.x >> 6) * (64 / TPT);
+  const bool wact = trw < R * NP2 && i0 + trw / NP2 < d.O0;
+  float2 v[16];
+  if (wact) {
+    TwSeed<TW> S;  // twiddle loads in flight with the row fetches
+    twiddles_seed<TW>(S, t, tw);
+    // the first three covering patch rows (all of them at 50 % overlap) are
+    // fetched together, any further ones one at a time; added in ascending
+    // patch-row order, absent rows as zeros
+    auto fetch = [&](float4(&a)[4], float4(&b)[4], int m) {
+      const int r = u - m * g.hop0;
+      const bool ok = m <= mhi && r >= 0 && r < d.srow;
+      const float2* ra = ok ? d.src + ((size_t)(kf + m * g.csf1 + na) * d.srow + r) * g.NBP : d.src;
+      c2r_fetch<TW>(a, b, ra, ra + (size_t)d.srow * g.NBP, ok, ok && hasb, t);
+    };
+    float4 A[4], B[4], a1[4], b1[4], a2[4], b2[4];
+    fetch(A, B, mlo);
+    fetch(a1, b1, mlo + 1);
+    fetch(a2, b2, mlo + 2);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      A[q].x += a1[q].x; A[q].y += a1[q].y; A[q].z += a1[q].z; A[q].w += a1[q].w;
+      B[q].x += b1[q].x; B[q].y += b1[q].y; B[q].z += b1[q].z; B[q].w += b1[q].w;
+      A[q].x += a2[q].x; A[q].y += a2[q].y; A[q].z += a2[q].z; A[q].w += a2[q].w;
+      B[q].x += b2[q].x; B[q].y += b2[q].y; B[q].z += b2[q].z; B[q].w += b2[q].w;
+    }
+    for (int m0 = 3; m0 < d.mc; ++m0) {
+      fetch(a1, b1, mlo + m0);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        A[q].x += a1[q].x; A[q].y += a1[q].y; A[q].z += a1[q].z; A[q].w += a1[q].w;
+        B[q].x += b1[q].x; B[q].y += b1[q].y; B[q].z += b1[q].z; B[q].w += b1[q].w;
+      }
+    }
+    Twiddles<TW> T;
+    twiddles_from_seed<TW>(T, S);
+    c2r_stage<TW>(v, A, B, sl, t);
+    fft<TW, +1>(v, sl, t, T, tw);
+  }
+  __syncthreads();  // the slices become class rows
+  {
+    float4* z4 = (float4*)rowbuf;
+    const int n4 = (R * NC * RLc) >> 2;
+    for (int e = threadIdx.x; e < n4; e += WG) z4[e] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+  __syncthreads();
+  if (wact && act) {
+    const int qa = na % NC;
+    const int qb = qa + 1 == NC ? 0 : qa + 1;
+    float* ba = rowbuf + (ri * NC + qa) * RLc + na * g.hop1;
+    float* bb = rowbuf + (ri * NC + qb) * RLc + nb * g.hop1;
+#pragma unroll
+    for (int mm = 0; mm < 16; ++mm) {
+      const int col = t + TPT * mm;
+      if (col < d.ext1) {
+        ba[col] = v[mm].x;
+        if (hasb) bb[col] = v[mm].y;
+      }
+    }
+  }
+  __syncthreads();
+  // four output columns per thread and step: their class reads are issued
+  // together
+  for (int rr = 0; rr < R && i0 + rr < d.O0; ++rr) {
+    const float* cr = rowbuf + rr * NC * RLc;
+    const size_t orow = ((size_t)fr * d.O0 + i0 + rr) * d.O1;
+    for (int j0 = threadIdx.x; j0 < d.O1; j0 += 4 * WG) {
+      float acc[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int vcol = j0 + e * WG + d.oc1;
+        acc[e] = 0.f;
+        if (j0 + e * WG < d.O1 && vcol >= 0 && vcol < RLc)
+          for (int q = 0; q < NC; ++q) acc[e] += cr[q * RLc + vcol];
+      }
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int j = j0 + e * WG;
+        if (j < d.O1) {
+          const size_t o = orow + j;
+          float a = acc[e] * d.scale;
+          if (d.sub) a -= d.sub[o];
+          if (d.mul) a *= d.mul[o];
+          if (d.axz) a += d.beta * d.axz[o];
+          if (d.clip_lo) a = fmaxf(a, d.lo);
+          d.out[o] = a;
+        }
+      }
+    }
+  }
+}
+
+// four workgroups per CU from 64 to 256 points (the LDS of the 256-point
+// form allows four): at most 128 VGPRs
+template <int TW>
+__global__ __launch_bounds__(WG, (TW >= 512 ? 3 : TW >= 64 ? 4 : 1)) void k_rows_c2r_sum(Geo g, OlaDst d,
+                                                                         const float2* __restrict__ tw) {
+  extern __shared__ float2 dyn[];
+  c2r_sum_body<TW>(g, d, (int)blockIdx.x * d.sumr, blockIdx.y, dyn, tw);
+}
+
 // float offset of the staged windows: past the FFT slices and the gather
 // buffer (which alias each other in single-round launches), rounded up to
 // 16 bytes for the windows' 16-byte loads
@@ -907,11 +1049,16 @@ __host__ __device__ int ola_window_offset_floats(const Geo& g, const OlaDst& d) 
 
 size_t ola_rowbuf_bytes(const Geo& g, const OlaDst& d) {
   const int TPW = WG / (g.Tw / 16);
+  if (d.sumr) return (size_t)d.sumr * d.ncls * ola_class_len(g, d) * sizeof(float);
   if (!d.stream && d.ncls) return (size_t)d.mc * d.ncls * ola_class_len(g, d) * sizeof(float);
   return (size_t)(d.stream ? 2 * TPW : d.mc * g.csf1) * d.ext1 * sizeof(float);
 }
 
 size_t ola_shared_bytes(const Geo& g, const OlaDst& d) {
+  if (d.sumr) {  // all slices of a workgroup, aliased by the class rows
+    const size_t fl = (size_t)(WG / (g.Tw / 16)) * (g.Tw + g.Tw / 16) * sizeof(float2);
+    return std::max(fl, ola_rowbuf_bytes(g, d));
+  }
   // staged window rows + slot table
   const size_t wl = d.win ? ((size_t)d.mc * d.nslot * ola_win_rw(g) + (size_t)d.mc * g.csf1) * sizeof(float) : 0;
   const size_t acc = d.stream ? (size_t)d.O1 * sizeof(float) : 0;
@@ -937,6 +1084,7 @@ void ola_layout(const Geo& g, OlaDst& d) {
   d.alias = ola_single_round(g, d);
   d.stream = ola_stream_rounds(g, d);
   d.ncls = 0;
+  d.sumr = 0;
   if (!d.alias || g.hop1 <= 0) return;
   OlaDst e = d;
   e.ncls = std::min(g.csf1, (d.ext1 + g.hop1 - 1) / g.hop1);
@@ -949,6 +1097,19 @@ void ola_layout(const Geo& g, OlaDst& d) {
   for (int wgs = 128; wgs <= WG; wgs += 64)
     if ((wgs == WG || ola_fits(g, d, wgs)) && waves(b1, wgs / 64) < waves(b0, wgs / 64)) return;
   d.ncls = e.ncls;
+  // without a window the covering patch rows can be summed before the
+  // transform: as many output rows per workgroup as its transforms hold,
+  // fewer if their class rows would cost resident wavefronts
+  if (d.win) return;
+  const int NP2 = (g.csf1 + 1) / 2;
+  for (int R = (WG / (g.Tw / 16)) / NP2; R >= 1; --R) {
+    OlaDst f = d;
+    f.sumr = R;
+    if (waves(ola_shared_bytes(g, f), WG / 64) >= waves(b1, WG / 64)) {
+      d.sumr = R;
+      return;
+    }
+  }
 }
 
 // threads per workgroup: a single-round launch whose grid fills the GPU
@@ -989,6 +1150,11 @@ static hipError_t c2r_ola_t(const Geo& g, const OlaDst& d, const float2* tw, hip
   const int frames = g.P / g.Pf;
   const size_t shm = ola_shared_bytes(g, d);
   ProfScope ps(d.win ? "rows_c2r_ola_win" : "rows_c2r_ola", st);
+  if (d.sumr) {  // summed form: 256 threads, sumr output rows per workgroup
+    const dim3 gs((unsigned)((d.O0 + d.sumr - 1) / d.sumr), (unsigned)frames);
+    hipLaunchKernelGGL(k_rows_c2r_sum<TW>, gs, dim3(WG), shm, st, g, d, tw);
+    return hipGetLastError();
+  }
   const dim3 grid((unsigned)d.O0, (unsigned)frames);
   if (wgs == 0) wgs = ola_wg(g, d, (long long)d.O0 * frames);
   if (wgs != WG && !ola_fits(g, d, wgs)) return hipErrorInvalidValue;
@@ -1159,6 +1325,15 @@ __global__ __launch_bounds__(WGS, (TW >= 512 ? 3 : 1)) void k_batch_ola(const Ba
   const int lb = (int)blockIdx.x - L.start[gi];
   const BatchGroup& B = G[L.gidx[gi]];
   const OlaDst& d = WIN ? B.ola_adj : B.ola_fwd;
+  if constexpr (!WIN && WGS == WG) {
+    if (d.sumr) {  // summed form: sumr output rows per workgroup
+      const int nbr = (d.O0 + d.sumr - 1) / d.sumr;
+      const int fs = lb / nbr;
+      if (fs >= B.g.P / B.g.Pf) return;
+      c2r_sum_body<TW>(B.g, d, (lb - fs * nbr) * d.sumr, fs, dyn, B.tw);
+      return;
+    }
+  }
   const int fr = lb / d.O0;
   if (fr >= B.g.P / B.g.Pf) return;
   if constexpr (WGS == WG) {
@@ -1197,7 +1372,9 @@ int batch_blocks(const BatchGroup& G, int pass) {
       const int CS = WG / (G.g.Th / 16);
       return G.g.P * ((G.g.NB - 1 + CS - 1) / CS);
     }
-    case BP_OLA_FWD: return G.ola_fwd.O0 * (G.g.P / G.g.Pf);
+    case BP_OLA_FWD:
+      return (G.ola_fwd.sumr ? (G.ola_fwd.O0 + G.ola_fwd.sumr - 1) / G.ola_fwd.sumr : G.ola_fwd.O0) *
+             (G.g.P / G.g.Pf);
     case BP_OLA_ADJ: return G.ola_adj.O0 * (G.g.P / G.g.Pf);
     case BP_STACK: return G.g.P * (((G.st_nrows + 1) / 2 + TPW - 1) / TPW);
     default: return 0;
@@ -1215,7 +1392,7 @@ int batch_ola_threads(const BatchGroup* H, const BatchLaunch& L, int pass) {
   for (int i = 0; i < L.ng; ++i) {
     const BatchGroup& B = H[L.gidx[i]];
     const OlaDst& d = pass == BP_OLA_ADJ ? B.ola_adj : B.ola_fwd;
-    if (!d.alias) return WG;
+    if (!d.alias || d.sumr) return WG;
     need = std::max(need, (B.g.Tw / 16) * d.mc * ((B.g.csf1 + 1) / 2));
   }
   return need <= 128 ? 128 : need <= 192 ? 192 : WG;
