@@ -114,6 +114,19 @@ void vmbd_plan_destroy(vmbd_plan* plan);
 int vmbd_plan_set_tuning(vmbd_plan* plan, const vmbd_tuning* tuning);
 int vmbd_plan_get_tuning(const vmbd_plan* plan, vmbd_tuning* tuning);
 
+/* Declare the plan's windows separable: every window class equals
+ * nom / wimg, with nom(r, c) = max(a[r] * b[c], floor) (a: sw0 values, b:
+ * sw1 values) and wimg the overlap-add of nom over the patch grid, as
+ * win2winaux builds them (imagetools.py:314-344, 438-445).  The plan checks
+ * that against its tables (relative 1e-5) and then runs the x-adjoint's
+ * overlap-add in summed form: sum_n b(c) IFFT(sum_m a(r) Y[m, n][r]) / wimg,
+ * with the pixels whose covering windows are all at the floor (the outer
+ * rows and columns of the patch grid) taken as the reference takes them,
+ * the mean of the covering patches.  VMBD_E_UNSUP when the tables do not
+ * factor (the plan keeps its two-dimensional windows).  Allocates; call
+ * once after vmbd_plan_create.  No counterpart in the reference's API. */
+int vmbd_plan_set_window_factors(vmbd_plan* plan, const float* a, const float* b, float floor);
+
 /* Device bytes held by the plan (workspaces, windows, twiddles). */
 size_t vmbd_plan_workspace_bytes(const vmbd_plan* plan);
 /* Output size of the forward operator for the plan's mode (olaGPU.py:127-136). */

@@ -339,3 +339,46 @@ def test_fused_solver_forms(cuda, exact):
         b = plan.adjoint_cached(kind, spec, gt, exact=exact, z=z, beta=0.37)
         ref = (a.double() + 0.37 * z.double()).cpu().numpy()
         assert rel_l2(b.cpu().numpy(), ref) < 1e-6, kind
+
+
+@pytest.mark.parametrize('geom', GEOMS, ids=lambda g: f'{g[0]}-{g[1]}-{g[2]}')
+def test_separable_windows(geom, cuda):
+    """The x-adjoint's summed form (separable windows) against the two-
+    dimensional window tables and the oracle, including the canvas's outer
+    rows and columns, where every covering window is at the 1e-16 floor and
+    the reference weight of each covering patch is 1/count."""
+    import copy
+    import torch
+    from vmbd_amd.imagetools import win2winaux
+    from vmbd_amd.ola import OlaPlan
+    sx, csf, sf, ov = geom
+    print(f'case separable sx={sx} csf={csf} sf={sf} ov={ov}', flush=True)
+    wa = win2winaux(sx, csf, ov)
+    owa = O.Win2WinAux(sx, csf, ov)
+    p_sep = OlaPlan(sx, sf, wa, device=cuda)
+    p_2d = OlaPlan(sx, sf, wa, device=cuda, separable=False)
+    assert p_sep.separable and not p_2d.separable
+    x, f, g = case_inputs(7, sx, csf, sf)
+    x32, f32, g32 = (np.float32(a).astype(np.float64) for a in (x, f, g))
+    xt, ft, gt = _t(x, cuda), _t(f, cuda), _t(g, cuda)
+    for exact in (True, False):
+        ref = O.adjoint_x(g32, f32, owa, sx, exact=exact)
+        a = p_sep.adjoint_x(gt, ft, exact=exact).cpu().numpy()
+        b = p_2d.adjoint_x(gt, ft, exact=exact).cpu().numpy()
+        assert rel_l2(a, ref) <= TOL and rel_l2(b, ref) <= TOL, exact
+        # outer rows and columns, pixel by pixel
+        scale = np.abs(ref).max()
+        edge = np.zeros(ref.shape, bool)
+        edge[[0, -1], :] = True
+        edge[:, [0, -1]] = True
+        assert np.abs(a - ref)[edge].max() <= 1e-5 * scale, exact
+        xb, fb = p_sep.adjoint_xf(gt, xt, ft, exact=exact)
+        assert rel_l2(xb.cpu().numpy(), ref) <= TOL
+    # tables that do not factor: the plan keeps its two-dimensional windows
+    wb = copy.copy(wa)
+    wb.win_table = wa.win_table.copy()
+    wb.win_table[0] *= np.float32(1.01)
+    p_bad = OlaPlan(sx, sf, wb, device=cuda)
+    p_bad2d = OlaPlan(sx, sf, wb, device=cuda, separable=False)
+    assert not p_bad.separable
+    assert torch.equal(p_bad.adjoint_x(gt, ft), p_bad2d.adjoint_x(gt, ft))

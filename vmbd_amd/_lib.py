@@ -46,6 +46,7 @@ SIGNATURES = {
     'vmbd_plan_destroy': (None, [_P]),
     'vmbd_plan_set_tuning': (ctypes.c_int, [_P, ctypes.POINTER(VmbdTuning)]),
     'vmbd_plan_get_tuning': (ctypes.c_int, [_P, ctypes.POINTER(VmbdTuning)]),
+    'vmbd_plan_set_window_factors': (ctypes.c_int, [_P, _P, _P, ctypes.c_float]),
     'vmbd_plan_workspace_bytes': (ctypes.c_size_t, [_P]),
     'vmbd_plan_out_shape': (ctypes.c_int, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
     'vmbd_plan_spectrum_elems': (ctypes.c_size_t, [_P]),

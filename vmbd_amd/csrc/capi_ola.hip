@@ -38,6 +38,15 @@ struct vmbd_plan {
   int max_frames = 1;
   int rows_frames = 0;  // >0: wsX/wsF hold the row transforms of the last forward (that many frames)
   vmbd_tuning tune{};   // launch shapes (0 = automatic)
+  // separable windows (vmbd_plan_set_window_factors): row and column factors,
+  // the x-adjoint's column normalisers [2][W] (1/sum of the covering column
+  // factors, 1/number of covering patch columns), rows whose covering
+  // windows are all at the floor [H], and the output columns where that holds
+  float* fac_a = nullptr;
+  float* fac_b = nullptr;
+  float* fac_colw = nullptr;
+  int* fac_zrow = nullptr;
+  int fac_bcol[2] = {-1, -1}, fac_bn[2] = {0, 0}, fac_bc[2] = {0, 0};
 };
 
 namespace {
@@ -77,6 +86,10 @@ void plan_free(vmbd_plan* p) {
   (void)hipFree(p->wsF);
   (void)hipFree(p->Y1);
   (void)hipFree(p->Y2);
+  (void)hipFree(p->fac_a);
+  (void)hipFree(p->fac_b);
+  (void)hipFree(p->fac_colw);
+  (void)hipFree(p->fac_zrow);
   delete p;
 }
 
@@ -228,6 +241,18 @@ OlaDst dst_adjoint_x(const vmbd_plan* p, float* xbar) {
   d.row_class = p->row_class;
   d.nslot = p->nslot;
   d.mc = p->mc_adj;
+  d.bcol[0] = d.bcol[1] = -1;
+  if (p->fac_a) {  // separable windows: summed form
+    d.wrow = p->fac_a;
+    d.wcol = p->fac_b;
+    d.colw = p->fac_colw;
+    d.zrow = p->fac_zrow;
+    for (int e = 0; e < 2; ++e) {
+      d.bcol[e] = p->fac_bcol[e];
+      d.bn[e] = p->fac_bn[e];
+      d.bc[e] = p->fac_bc[e];
+    }
+  }
   ola_layout(g, d);
   return d;
 }
@@ -434,6 +459,125 @@ int vmbd_plan_set_tuning(vmbd_plan* p, const vmbd_tuning* t) {
                                   ola_fits(p->g, dst_adjoint_x(p, nullptr), o)))
     return vmbd_fail(VMBD_E_UNSUP, "vmbd_plan_set_tuning: an output row needs more than one round of %d threads", o);
   p->tune = *t;
+  return VMBD_OK;
+}
+
+int vmbd_plan_set_window_factors(vmbd_plan* p, const float* a, const float* b, float floor_) {
+  vmbd_clear_error();
+  if (!p || !a || !b) return vmbd_fail(VMBD_E_ARG, "vmbd_plan_set_window_factors: null argument");
+  if (!(floor_ >= 0.f)) return vmbd_fail(VMBD_E_ARG, "vmbd_plan_set_window_factors: floor must be >= 0");
+  if (p->fac_a) return vmbd_fail(VMBD_E_ARG, "vmbd_plan_set_window_factors: factors already set");
+  const Geo& g = p->g;
+  const int sw0 = g.sw0, sw1 = g.sw1, hop0 = g.hop0, hop1 = g.hop1;
+  const int S0 = (g.csf0 - 1) * hop0 + sw0, S1 = (g.csf1 - 1) * hop1 + sw1;  // overlap-add canvas
+  // nom and its overlap-add in FP64
+  std::vector<double> nom((size_t)sw0 * sw1), wimg((size_t)S0 * S1, 0.0);
+  for (int r = 0; r < sw0; ++r)
+    for (int c = 0; c < sw1; ++c) nom[(size_t)r * sw1 + c] = std::max((double)a[r] * (double)b[c], (double)floor_);
+  for (int m = 0; m < g.csf0; ++m)
+    for (int n = 0; n < g.csf1; ++n)
+      for (int r = 0; r < sw0; ++r)
+        for (int c = 0; c < sw1; ++c) wimg[(size_t)(m * hop0 + r) * S1 + n * hop1 + c] += nom[(size_t)r * sw1 + c];
+  // the plan's windows must be nom / wimg
+  std::vector<float> tab((size_t)p->n_win * sw0 * sw1);
+  std::vector<int> idx(g.Pf);
+  hipError_t e = hipMemcpy(tab.data(), p->win, tab.size() * sizeof(float), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(idx.data(), p->win_idx, idx.size() * sizeof(int), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return vmbd_fail_hip("vmbd_plan_set_window_factors: hipMemcpy", e);
+  for (int k = 0; k < g.Pf; ++k) {
+    const int m = k / g.csf1, n = k - m * g.csf1;
+    const float* w = tab.data() + (size_t)idx[k] * sw0 * sw1;
+    for (int r = 0; r < sw0; ++r)
+      for (int c = 0; c < sw1; ++c) {
+        const double ref = nom[(size_t)r * sw1 + c] / wimg[(size_t)(m * hop0 + r) * S1 + n * hop1 + c];
+        if (!(std::fabs((double)w[(size_t)r * sw1 + c] - ref) <= 1e-5 * std::fabs(ref) + 1e-30))
+          return vmbd_fail(VMBD_E_UNSUP,
+                           "vmbd_plan_set_window_factors: window of patch %d differs from nom/wimg at (%d, %d)", k,
+                           r, c);
+      }
+  }
+  // a row (column) of the canvas is at the floor when every covering patch
+  // row (column) has a zero factor there; a pixel on such a row or column
+  // has the reference weight 1/count for each covering patch
+  auto floor_line = [](const float* f, int csf, int hop, int sw, int v) {
+    int cover = 0;
+    for (int m = 0; m < csf; ++m) {
+      const int r = v - m * hop;
+      if (r < 0 || r >= sw) continue;
+      ++cover;
+      if (f[r] != 0.f) return false;
+    }
+    return cover > 0;
+  };
+  const int H = g.H, W = g.W;  // x-adjoint output: canvas rows [0, H), columns [0, W)
+  std::vector<int> zrow(H);
+  for (int u = 0; u < H; ++u) zrow[u] = u < S0 && floor_line(a, g.csf0, hop0, sw0, u);
+  int nb = 0, bcol[2] = {-1, -1}, bn[2] = {0, 0}, bc[2] = {0, 0};
+  for (int v = 0; v < W && v < S1; ++v) {
+    if (!floor_line(b, g.csf1, hop1, sw1, v)) continue;
+    int cover = 0, nn = 0;
+    for (int n = 0; n < g.csf1; ++n)
+      if (v - n * hop1 >= 0 && v - n * hop1 < sw1) {
+        ++cover;
+        nn = n;
+      }
+    if (nb == 2 || cover != 1)
+      return vmbd_fail(VMBD_E_UNSUP, "vmbd_plan_set_window_factors: floor column %d not at the canvas edge", v);
+    bcol[nb] = v;
+    bn[nb] = nn;
+    bc[nb] = v - nn * hop1;
+    ++nb;
+  }
+  // wimg(u, v) = A(u) B(v) away from the floor (A, B: sums of the covering
+  // row and column factors); on a floor row the weights are 1/count =
+  // 1/(rows * columns covering the pixel).  Per output column: 1/B(v) and
+  // 1/(number of covering patch columns)
+  std::vector<float> colw((size_t)2 * W, 0.f);
+  for (int v = 0; v < W && v < S1; ++v) {
+    double B = 0.0;
+    int cn = 0;
+    for (int n = 0; n < g.csf1; ++n) {
+      const int c = v - n * hop1;
+      if (c < 0 || c >= sw1) continue;
+      B += b[c];
+      ++cn;
+    }
+    colw[v] = B > 0.0 ? (float)(1.0 / B) : 0.f;
+    colw[(size_t)W + v] = cn ? (float)(1.0 / cn) : 0.f;
+  }
+  size_t tot = 0;
+  if (!(dev_alloc(&p->fac_a, sw0, tot) && dev_alloc(&p->fac_b, sw1, tot) && dev_alloc(&p->fac_colw, colw.size(), tot) &&
+        dev_alloc(&p->fac_zrow, zrow.size(), tot))) {
+    (void)hipFree(p->fac_a);
+    (void)hipFree(p->fac_b);
+    (void)hipFree(p->fac_colw);
+    (void)hipFree(p->fac_zrow);
+    p->fac_a = p->fac_b = p->fac_colw = nullptr;
+    p->fac_zrow = nullptr;
+    return vmbd_fail(VMBD_E_NOMEM, "vmbd_plan_set_window_factors: hipMalloc failed");
+  }
+  e = hipMemcpy(p->fac_a, a, sw0 * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(p->fac_b, b, sw1 * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(p->fac_colw, colw.data(), colw.size() * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(p->fac_zrow, zrow.data(), zrow.size() * sizeof(int), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return vmbd_fail_hip("vmbd_plan_set_window_factors: hipMemcpy", e);
+  for (int e2 = 0; e2 < 2; ++e2) {
+    p->fac_bcol[e2] = bcol[e2];
+    p->fac_bn[e2] = bn[e2];
+    p->fac_bc[e2] = bc[e2];
+  }
+  p->bytes += tot;
+  // the summed form's LDS must fit (it needs less than the per-patch form)
+  if (ola_shared_bytes(g, dst_adjoint_x(p, nullptr)) > 160 * 1024) {
+    (void)hipFree(p->fac_a);
+    (void)hipFree(p->fac_b);
+    (void)hipFree(p->fac_colw);
+    (void)hipFree(p->fac_zrow);
+    p->fac_a = p->fac_b = p->fac_colw = nullptr;
+    p->fac_zrow = nullptr;
+    p->bytes -= tot;
+    return vmbd_fail(VMBD_E_UNSUP, "vmbd_plan_set_window_factors: summed x-adjoint does not fit the LDS");
+  }
   return VMBD_OK;
 }
 
@@ -716,6 +860,9 @@ OlaDst dst_patches(const vmbd_plan* p, float* out, bool window) {
     d.win_slot = nullptr;
     d.row_class = nullptr;
     d.nslot = 0;
+    d.wrow = d.wcol = d.colw = nullptr;
+    d.zrow = nullptr;
+    d.bcol[0] = d.bcol[1] = -1;
   }
   return d;
 }

@@ -112,6 +112,19 @@ struct OlaDst {
   // the row-frequency domain before one inverse transform per patch-column
   // pair, sumr output rows per 256-thread workgroup (ola_layout)
   int sumr;
+  // separable windows of the summed form (null: unit weights): row weight
+  // wrow[r], column weight wcol[c] of patch-local (r, c); per output column
+  // j the normalisers colw[j] = 1 / sum of the covering b(c) and
+  // colw[O1 + j] = 1 / number of covering patch columns; zrow[i] != 0: every
+  // covering window is at the floor on row i (unit weights); output columns
+  // bcol[e] (-1: none, OLA coordinates) are covered by patch column bn[e]
+  // alone, at local column bc[e], where its window is at the floor: their
+  // values in the other rows are the unweighted sums, evaluated apart
+  const float* wrow;
+  const float* wcol;
+  const float* colw;
+  const int* zrow;
+  int bcol[2], bn[2], bc[2];
   // 1/(Th*Tw) of the unnormalised transforms, applied to the overlap-add sum
   float scale;
   // fused solver arithmetic on the stored value v = scale * sum (each optional):

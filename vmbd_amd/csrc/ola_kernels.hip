@@ -892,19 +892,34 @@ __global__ __launch_bounds__(WGS, (TW >= 512 ? 3 : 1)) void k_rows_c2r_ola(Geo g
   c2r_ola_body<TW, STREAM, WIN, WGS>(g, d, blockIdx.x, blockIdx.y, dyn, tw);
 }
 
-// Summed form of the overlap-add pass (no window, class layout): the inverse
-// row transform is linear and the patches (m, n) of one patch column n land
-// on the same output columns n*hop1 + c, so output row u is
+// Summed form of the overlap-add pass (class layout).  The inverse row
+// transform is linear and the patches (m, n) of one patch column n land on
+// the same output columns n*hop1 + c, so output row u is
 //   sum_n IFFT_row( sum_m Y[m, n][u - m*hop0] )  at columns n*hop1 + c:
 // one transform per (output row, pair of patch columns) instead of one per
 // (covering patch row, pair), 2.4x fewer at 50 % overlap.  R = d.sumr output
 // rows per workgroup share its TPW transforms (R * NP2 <= TPW).  A transform
-// fetches its covering rows two patch rows at a time (absent rows read the
-// zero row) and adds them in ascending patch-row order; its two patches' rows
-// go to their class rows (patch n to row n mod ncls of its output row), and
-// each output column adds its ncls class values.  Output rows i0 ..
-// i0 + R - 1 of frame fr (dyn: ola_shared_bytes of LDS).
-template <int TW>
+// fetches its covering rows (the first three together, absent rows read the
+// zero row) and adds them in ascending patch-row order; its two patches'
+// rows go to their class rows (patch n to row n mod ncls of its output row),
+// and each output column adds its ncls class values.
+//
+// WGT: separable windows nom(r, c) = a(r) b(c) over the normaliser wimg (the
+// x-adjoint; olaGPU.py:305-309 applies ws = nom / wimg per patch): row m
+// enters the sum weighted by a(r), the transform's column c by b(c), the
+// output pixel by 1/wimg.  Where every covering window is at the floor (rows
+// with zrow set, columns bcol) the reference's weights are 1/count: such a
+// row takes unit weights (its normaliser is 1/count); such a column of the
+// other rows is the unweighted sum of its covering patches' inverse
+// transforms at that column, evaluated as one inverse DFT term per bin.
+// Output rows i0 .. i0 + R - 1 of frame fr (dyn: ola_shared_bytes of LDS).
+__host__ __device__ inline int ola_sum_wcol_offset_floats(const Geo& g, const OlaDst& d) {
+  const int fl = (WG / (g.Tw / 16)) * (g.Tw + g.Tw / 16) * 2;
+  const int rb = d.sumr * d.ncls * ola_class_len(g, d);
+  return ((fl > rb ? fl : rb) + 3) & ~3;
+}
+
+template <int TW, bool WGT>
 __device__ __forceinline__ void c2r_sum_body(Geo g, OlaDst d, int i0, int fr, float2* dyn,
                                              const float2* __restrict__ tw) {
   constexpr int TPT = TW / 16;
@@ -932,39 +947,102 @@ __device__ __forceinline__ void c2r_sum_body(Geo g, OlaDst d, int i0, int fr, fl
   const bool hasb = nb < g.csf1;
   const int kf = fr * g.Pf;
   float2* sl = dyn + tr * SL;
+  // column weights staged in thread order (thread t weights columns
+  // t + TPT*mm, kept at floats t*MW + mm): 16-byte LDS reads
+  const int MW = ola_win_mw(g);
+  float* wcs = (float*)dyn + ola_sum_wcol_offset_floats(g, d);
+  if constexpr (WGT) {
+    for (int e = threadIdx.x; e < TPT * MW; e += WG) {
+      const int tt = e / MW;
+      const int c = tt + TPT * (e - tt * MW);
+      wcs[e] = c < g.sw1 ? d.wcol[c] : 0.f;
+    }
+  }
+  // unit weights on a row whose covering windows are all at the floor
+  const bool zr = WGT && act && d.zrow[i];
   // a wavefront whose transforms all lie past the workgroup's rows skips them
   const int trw = (int)(threadIdx.x >> 6) * (64 / TPT);
   const bool wact = trw < R * NP2 && i0 + trw / NP2 < d.O0;
   float2 v[16];
+  float invA = 1.f;
+  float s0 = 0.f;
+  bool f0a = false;
   if (wact) {
     TwSeed<TW> S;  // twiddle loads in flight with the row fetches
     twiddles_seed<TW>(S, t, tw);
     // the first three covering patch rows (all of them at 50 % overlap) are
     // fetched together, any further ones one at a time; added in ascending
-    // patch-row order, absent rows as zeros
+    // patch-row order, absent rows as zeros.  Returns the row's weight.
+    int nok = 0;  // covering rows fetched
     auto fetch = [&](float4(&a)[4], float4(&b)[4], int m) {
       const int r = u - m * g.hop0;
       const bool ok = m <= mhi && r >= 0 && r < d.srow;
       const float2* ra = ok ? d.src + ((size_t)(kf + m * g.csf1 + na) * d.srow + r) * g.NBP : d.src;
       c2r_fetch<TW>(a, b, ra, ra + (size_t)d.srow * g.NBP, ok, ok && hasb, t);
+      nok += ok;
+      return ok ? ((WGT && !zr) ? d.wrow[r] : 1.f) : 0.f;
     };
-    float4 A[4], B[4], a1[4], b1[4], a2[4], b2[4];
-    fetch(A, B, mlo);
-    fetch(a1, b1, mlo + 1);
-    fetch(a2, b2, mlo + 2);
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      A[q].x += a1[q].x; A[q].y += a1[q].y; A[q].z += a1[q].z; A[q].w += a1[q].w;
-      B[q].x += b1[q].x; B[q].y += b1[q].y; B[q].z += b1[q].z; B[q].w += b1[q].w;
-      A[q].x += a2[q].x; A[q].y += a2[q].y; A[q].z += a2[q].z; A[q].w += a2[q].w;
-      B[q].x += b2[q].x; B[q].y += b2[q].y; B[q].z += b2[q].z; B[q].w += b2[q].w;
-    }
-    for (int m0 = 3; m0 < d.mc; ++m0) {
-      fetch(a1, b1, mlo + m0);
+    auto add = [&](float4(&A)[4], float4(&B)[4], const float4(&a)[4], const float4(&b)[4], float w) {
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        A[q].x += a1[q].x; A[q].y += a1[q].y; A[q].z += a1[q].z; A[q].w += a1[q].w;
-        B[q].x += b1[q].x; B[q].y += b1[q].y; B[q].z += b1[q].z; B[q].w += b1[q].w;
+        if constexpr (WGT) {
+          A[q].x = fmaf(w, a[q].x, A[q].x); A[q].y = fmaf(w, a[q].y, A[q].y);
+          A[q].z = fmaf(w, a[q].z, A[q].z); A[q].w = fmaf(w, a[q].w, A[q].w);
+          B[q].x = fmaf(w, b[q].x, B[q].x); B[q].y = fmaf(w, b[q].y, B[q].y);
+          B[q].z = fmaf(w, b[q].z, B[q].z); B[q].w = fmaf(w, b[q].w, B[q].w);
+        } else {
+          A[q].x += a[q].x; A[q].y += a[q].y; A[q].z += a[q].z; A[q].w += a[q].w;
+          B[q].x += b[q].x; B[q].y += b[q].y; B[q].z += b[q].z; B[q].w += b[q].w;
+        }
+      }
+    };
+    // the weighted form (the x-adjoint: two covering patch rows at 50 %
+    // overlap) fetches two rows together, which keeps it within 128 VGPRs
+    constexpr int NPEEL = WGT ? 2 : 3;
+    // a patch of this pair whose local column 0 is a floor column of the
+    // canvas (left edge): its unweighted inverse transforms at column 0,
+    // X[0] + X[Tw/2] + 2 sum_{0<k<Tw/2} Re X[k], summed over the covering
+    // rows from the fetched bins
+    f0a = WGT && !zr && ((d.bcol[0] >= 0 && d.bc[0] == 0 && d.bn[0] == na) ||
+                         (d.bcol[1] >= 0 && d.bc[1] == 0 && d.bn[1] == na));
+    auto edge0 = [&](const float4(&a)[4]) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        s0 += (t == 0 && q == 0) ? (a[q].x + a[q].y) + 2.f * a[q].z : 2.f * (a[q].x + a[q].z);
+    };
+    float4 A[4], B[4], a1[4], b1[4], a2[4], b2[4];
+    const float w0 = fetch(A, B, mlo);
+    const float w1 = fetch(a1, b1, mlo + 1);
+    float w2 = 0.f;
+    if (NPEEL == 3 && d.mc >= 3) w2 = fetch(a2, b2, mlo + 2);
+    if constexpr (WGT) {
+      if (f0a) {
+        edge0(A);
+        edge0(a1);
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        A[q].x *= w0; A[q].y *= w0; A[q].z *= w0; A[q].w *= w0;
+        B[q].x *= w0; B[q].y *= w0; B[q].z *= w0; B[q].w *= w0;
+      }
+    }
+    add(A, B, a1, b1, w1);
+    if (NPEEL == 3 && d.mc >= 3) add(A, B, a2, b2, w2);
+    float wsum = w0 + w1 + w2;
+    for (int m0 = NPEEL; m0 < d.mc; ++m0) {
+      const float wk = fetch(a1, b1, mlo + m0);
+      if (WGT && f0a) edge0(a1);
+      add(A, B, a1, b1, wk);
+      wsum += wk;
+    }
+    if constexpr (WGT) {
+      // 1/(sum of the row weights): the row part of 1/wimg (1/count on a
+      // floor row, whose weights are 1)
+      invA = wsum > 0.f ? 1.f / wsum : 0.f;
+      if (f0a) {  // the reference weight there: 1/count, one covering patch column
+#pragma unroll
+        for (int off = 1; off < TPT; off <<= 1) s0 += __shfl_xor(s0, off, TPT);
+        s0 = nok ? s0 / (float)nok : 0.f;
       }
     }
     Twiddles<TW> T;
@@ -972,7 +1050,7 @@ __device__ __forceinline__ void c2r_sum_body(Geo g, OlaDst d, int i0, int fr, fl
     c2r_stage<TW>(v, A, B, sl, t);
     fft<TW, +1>(v, sl, t, T, tw);
   }
-  __syncthreads();  // the slices become class rows
+  __syncthreads();  // the slices become class rows; column weights staged
   {
     float4* z4 = (float4*)rowbuf;
     const int n4 = (R * NC * RLc) >> 2;
@@ -984,21 +1062,36 @@ __device__ __forceinline__ void c2r_sum_body(Geo g, OlaDst d, int i0, int fr, fl
     const int qb = qa + 1 == NC ? 0 : qa + 1;
     float* ba = rowbuf + (ri * NC + qa) * RLc + na * g.hop1;
     float* bb = rowbuf + (ri * NC + qb) * RLc + nb * g.hop1;
+    const float* wct = wcs + t * MW;
+    float4 qw = make_float4(1.f, 1.f, 1.f, 1.f);
 #pragma unroll
     for (int mm = 0; mm < 16; ++mm) {
       const int col = t + TPT * mm;
       if (col < d.ext1) {
-        ba[col] = v[mm].x;
-        if (hasb) bb[col] = v[mm].y;
+        float xa = v[mm].x, xb = v[mm].y;
+        if constexpr (WGT) {
+          if ((mm & 3) == 0 && !zr) qw = *(const float4*)(wct + mm);
+          const float wc = ((mm & 3) == 0 ? qw.x : (mm & 3) == 1 ? qw.y : (mm & 3) == 2 ? qw.z : qw.w) * invA;
+          xa *= wc;
+          xb *= wc;
+        }
+        ba[col] = xa;
+        if (hasb) bb[col] = xb;
       }
     }
+    if (WGT && f0a && t == 0) wcs[TPT * MW + ri] = s0;
   }
   __syncthreads();
   // four output columns per thread and step: their class reads are issued
   // together
   for (int rr = 0; rr < R && i0 + rr < d.O0; ++rr) {
     const float* cr = rowbuf + rr * NC * RLc;
-    const size_t orow = ((size_t)fr * d.O0 + i0 + rr) * d.O1;
+    const int irow = i0 + rr;
+    const size_t orow = ((size_t)fr * d.O0 + irow) * d.O1;
+    // floor columns of a row with real weights: the left one from its slot,
+    // others evaluated apart below
+    const bool fix = WGT && !d.zrow[irow];
+    const float* cw = WGT ? d.colw + (fix ? 0 : d.O1) : nullptr;
     for (int j0 = threadIdx.x; j0 < d.O1; j0 += 4 * WG) {
       float acc[4];
 #pragma unroll
@@ -1011,9 +1104,19 @@ __device__ __forceinline__ void c2r_sum_body(Geo g, OlaDst d, int i0, int fr, fl
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const int j = j0 + e * WG;
-        if (j < d.O1) {
+        const int vcol = j + d.oc1;
+        // (selects, not indexing: a runtime index would put d in scratch)
+        const int eb = vcol == d.bcol[0] ? 0 : vcol == d.bcol[1] ? 1 : -1;
+        const int bce = eb == 0 ? d.bc[0] : d.bc[1];
+        if (j < d.O1 && !(fix && eb >= 0 && bce != 0)) {
           const size_t o = orow + j;
-          float a = acc[e] * d.scale;
+          float a;
+          if (WGT && fix && eb >= 0)
+            a = wcs[TPT * MW + rr] * d.scale;  // left floor column: mean of the covering patches
+          else if constexpr (WGT)
+            a = acc[e] * d.scale * cw[j];
+          else
+            a = acc[e] * d.scale;
           if (d.sub) a -= d.sub[o];
           if (d.mul) a *= d.mul[o];
           if (d.axz) a += d.beta * d.axz[o];
@@ -1023,15 +1126,64 @@ __device__ __forceinline__ void c2r_sum_body(Geo g, OlaDst d, int i0, int fr, fl
       }
     }
   }
+  if constexpr (WGT) {
+    // floor columns of the rows with real weights: sum over the covering
+    // patch rows m of patch column bn's inverse transform at column bc,
+    // x(c) = X[0] + (-1)^c X[Tw/2] + 2 sum_{0<k<Tw/2} Re(X[k] e^{2 pi i k c / Tw}),
+    // one wavefront per (row, column), bins over the lanes, a fixed-order
+    // butterfly reduction
+    const int lane = threadIdx.x & 63;
+    for (int jb = (int)(threadIdx.x >> 6); jb < 2 * R; jb += WG / 64) {
+      const int rr = jb >> 1, e = jb & 1;
+      const int irow = i0 + rr;
+      const int bcol = e ? d.bcol[1] : d.bcol[0], bn = e ? d.bn[1] : d.bn[0], bc = e ? d.bc[1] : d.bc[0];
+      if (irow >= d.O0 || bcol < 0 || bc == 0 || d.zrow[irow]) continue;
+      const int j = bcol - d.oc1;
+      if (j < 0 || j >= d.O1) continue;
+      const int uu = irow + d.oc0;
+      const int mh = min(g.csf0 - 1, uu / g.hop0);
+      const int l0 = uu - d.ext0 + 1;
+      const int ml = l0 <= 0 ? 0 : (l0 + g.hop0 - 1) / g.hop0;
+      const int c = bc;
+      float sacc = 0.f;
+      int cnt = 0;
+      for (int m = ml; m <= mh; ++m) {
+        const int r = uu - m * g.hop0;
+        if (r < 0 || r >= d.srow) continue;
+        ++cnt;
+        const float2* row = d.src + ((size_t)(kf + m * g.csf1 + bn) * d.srow + r) * g.NBP;
+        for (int k = lane; k < TW / 2; k += 64) {
+          const float2 X = row[k];
+          if (k == 0) {
+            sacc += X.x + ((c & 1) ? -X.y : X.y);
+          } else {
+            const float2 w = tw[(k * c * (TW_LEN / TW)) & (TW_LEN - 1)];
+            sacc += 2.f * fmaf(X.x, w.x, X.y * w.y);
+          }
+        }
+      }
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) sacc += __shfl_xor(sacc, off, 64);
+      if (lane == 0) {
+        const size_t o = ((size_t)fr * d.O0 + irow) * d.O1 + j;
+        float a = cnt ? sacc * d.scale / (float)cnt : 0.f;
+        if (d.sub) a -= d.sub[o];
+        if (d.mul) a *= d.mul[o];
+        if (d.axz) a += d.beta * d.axz[o];
+        if (d.clip_lo) a = fmaxf(a, d.lo);
+        d.out[o] = a;
+      }
+    }
+  }
 }
 
 // four workgroups per CU from 64 to 256 points (the LDS of the 256-point
 // form allows four): at most 128 VGPRs
-template <int TW>
+template <int TW, bool WGT>
 __global__ __launch_bounds__(WG, (TW >= 512 ? 3 : TW >= 64 ? 4 : 1)) void k_rows_c2r_sum(Geo g, OlaDst d,
                                                                          const float2* __restrict__ tw) {
   extern __shared__ float2 dyn[];
-  c2r_sum_body<TW>(g, d, (int)blockIdx.x * d.sumr, blockIdx.y, dyn, tw);
+  c2r_sum_body<TW, WGT>(g, d, (int)blockIdx.x * d.sumr, blockIdx.y, dyn, tw);
 }
 
 // float offset of the staged windows: past the FFT slices and the gather
@@ -1055,10 +1207,10 @@ size_t ola_rowbuf_bytes(const Geo& g, const OlaDst& d) {
 }
 
 size_t ola_shared_bytes(const Geo& g, const OlaDst& d) {
-  if (d.sumr) {  // all slices of a workgroup, aliased by the class rows
-    const size_t fl = (size_t)(WG / (g.Tw / 16)) * (g.Tw + g.Tw / 16) * sizeof(float2);
-    return std::max(fl, ola_rowbuf_bytes(g, d));
-  }
+  if (d.sumr)  // all slices of a workgroup, aliased by the class rows; staged column
+               // weights and one left-floor value per output row
+    return (size_t)ola_sum_wcol_offset_floats(g, d) * sizeof(float) +
+           (d.wcol ? ((size_t)ola_win_rw(g) + d.sumr) * sizeof(float) : 0);
   // staged window rows + slot table
   const size_t wl = d.win ? ((size_t)d.mc * d.nslot * ola_win_rw(g) + (size_t)d.mc * g.csf1) * sizeof(float) : 0;
   const size_t acc = d.stream ? (size_t)d.O1 * sizeof(float) : 0;
@@ -1100,7 +1252,7 @@ void ola_layout(const Geo& g, OlaDst& d) {
   // without a window the covering patch rows can be summed before the
   // transform: as many output rows per workgroup as its transforms hold,
   // fewer if their class rows would cost resident wavefronts
-  if (d.win) return;
+  if (d.win && !d.wrow) return;
   const int NP2 = (g.csf1 + 1) / 2;
   for (int R = (WG / (g.Tw / 16)) / NP2; R >= 1; --R) {
     OlaDst f = d;
@@ -1152,7 +1304,10 @@ static hipError_t c2r_ola_t(const Geo& g, const OlaDst& d, const float2* tw, hip
   ProfScope ps(d.win ? "rows_c2r_ola_win" : "rows_c2r_ola", st);
   if (d.sumr) {  // summed form: 256 threads, sumr output rows per workgroup
     const dim3 gs((unsigned)((d.O0 + d.sumr - 1) / d.sumr), (unsigned)frames);
-    hipLaunchKernelGGL(k_rows_c2r_sum<TW>, gs, dim3(WG), shm, st, g, d, tw);
+    if (d.wrow)
+      hipLaunchKernelGGL((k_rows_c2r_sum<TW, true>), gs, dim3(WG), shm, st, g, d, tw);
+    else
+      hipLaunchKernelGGL((k_rows_c2r_sum<TW, false>), gs, dim3(WG), shm, st, g, d, tw);
     return hipGetLastError();
   }
   const dim3 grid((unsigned)d.O0, (unsigned)frames);
@@ -1325,12 +1480,15 @@ __global__ __launch_bounds__(WGS, (TW >= 512 ? 3 : 1)) void k_batch_ola(const Ba
   const int lb = (int)blockIdx.x - L.start[gi];
   const BatchGroup& B = G[L.gidx[gi]];
   const OlaDst& d = WIN ? B.ola_adj : B.ola_fwd;
-  if constexpr (!WIN && WGS == WG) {
+  if constexpr (WGS == WG) {
     if (d.sumr) {  // summed form: sumr output rows per workgroup
       const int nbr = (d.O0 + d.sumr - 1) / d.sumr;
       const int fs = lb / nbr;
       if (fs >= B.g.P / B.g.Pf) return;
-      c2r_sum_body<TW>(B.g, d, (lb - fs * nbr) * d.sumr, fs, dyn, B.tw);
+      if (d.wrow)
+        c2r_sum_body<TW, true>(B.g, d, (lb - fs * nbr) * d.sumr, fs, dyn, B.tw);
+      else
+        c2r_sum_body<TW, false>(B.g, d, (lb - fs * nbr) * d.sumr, fs, dyn, B.tw);
       return;
     }
   }
@@ -1375,7 +1533,9 @@ int batch_blocks(const BatchGroup& G, int pass) {
     case BP_OLA_FWD:
       return (G.ola_fwd.sumr ? (G.ola_fwd.O0 + G.ola_fwd.sumr - 1) / G.ola_fwd.sumr : G.ola_fwd.O0) *
              (G.g.P / G.g.Pf);
-    case BP_OLA_ADJ: return G.ola_adj.O0 * (G.g.P / G.g.Pf);
+    case BP_OLA_ADJ:
+      return (G.ola_adj.sumr ? (G.ola_adj.O0 + G.ola_adj.sumr - 1) / G.ola_adj.sumr : G.ola_adj.O0) *
+             (G.g.P / G.g.Pf);
     case BP_STACK: return G.g.P * (((G.st_nrows + 1) / 2 + TPW - 1) / TPW);
     default: return 0;
   }

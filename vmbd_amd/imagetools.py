@@ -124,6 +124,15 @@ class win2winaux:
         self._device = device
         self._cache = {}
 
+    def window_factors(self):
+        """(a, b, floor): the window is max(a[r] * b[c], floor) (the outer
+        product of the 1-D factors scaled to max 1, as in ``window``)."""
+        w1 = window1d(int(self.sw[0]))
+        w2 = window1d(int(self.sw[1]))
+        a = np.ascontiguousarray(w1 / w1.max(), dtype=np.float32)
+        b = np.ascontiguousarray(w2 / w2.max(), dtype=np.float32)
+        return a, b, WINDOW_FLOOR
+
     @property
     def P(self):
         return self.csf[0] * self.csf[1]

@@ -42,7 +42,7 @@ def _as_dev(t, device, name):
 
 
 class OlaPlan:
-    def __init__(self, sx, sf, winaux, mode='valid', tile=None, device=None, max_frames=1):
+    def __init__(self, sx, sf, winaux, mode='valid', tile=None, device=None, max_frames=1, separable=True):
         if mode not in _lib.MODES:
             raise NotImplementedError('Not a valid mode!')
         sx = tuple(int(v) for v in sx)
@@ -75,6 +75,19 @@ class OlaPlan:
         self.spec_elems = int(L.vmbd_plan_spectrum_elems(self._h))
         self.workspace_bytes = int(L.vmbd_plan_workspace_bytes(self._h))
         self._rows_key = None
+        # separable windows: the x-adjoint's overlap-add in summed form (the
+        # library checks the factors against the tables and refuses otherwise)
+        self.separable = False
+        if separable and hasattr(winaux, 'window_factors'):
+            fa, fb, floor = winaux.window_factors()
+            with torch.cuda.device(self.device):
+                rc = L.vmbd_plan_set_window_factors(self._h, fa.ctypes.data_as(ctypes.c_void_p),
+                                                    fb.ctypes.data_as(ctypes.c_void_p), float(floor))
+            if rc == 0:
+                self.separable = True
+                self.workspace_bytes = int(L.vmbd_plan_workspace_bytes(self._h))
+            elif rc != _lib.VMBD_E_UNSUP:
+                _lib.check(rc, 'vmbd_plan_set_window_factors')
 
     # -- launch shapes --------------------------------------------------------
     def set_tuning(self, ola_threads=0, col_threads=0):
