@@ -1029,6 +1029,7 @@ BatchGroup batch_group(vmbd_plan* p, int nframes, const float* x, const float* f
   B.cf_b = raw_in(p->wsF, sf);
   B.cf_o = ColOut{p->Y1, g.ext0};
   B.ola_fwd = dst_forward(p, y);
+  ola_fit_rows(B.ola_fwd, (long long)B.ola_fwd.O0 * nframes);
   const SharedRows sr = residual_rows(p, g, gres, exact);
   B.gres = sr.g;
   B.rows_res.n = 1;
@@ -1041,6 +1042,7 @@ BatchGroup batch_group(vmbd_plan* p, int nframes, const float* x, const float* f
   B.ca_o1 = ColOut{p->Y1, g.sw0};
   B.ca_o2 = ColOut{p->Y2, g.sf0};
   B.ola_adj = dst_adjoint_x(p, xbar);
+  ola_fit_rows(B.ola_adj, (long long)B.ola_adj.O0 * nframes);
   // the grouped OLA launch runs WG threads per workgroup for every group
   B.st_src = p->Y2;
   B.st_srow = g.sf0;

@@ -230,6 +230,9 @@ __host__ __device__ inline int ola_class_len(const Geo& g, const OlaDst& d) {
 }
 // sets d.alias, d.stream and d.ncls of a destination whose other fields are set
 void ola_layout(const Geo& g, OlaDst& d);
+// fewer summed-form rows per workgroup for a launch of `rows` output rows
+// that would not fill the GPU several times over
+void ola_fit_rows(OlaDst& d, long long rows);
 size_t ola_shared_bytes(const Geo& g, const OlaDst& d);
 int ola_single_round(const Geo& g, const OlaDst& d);
 // multi-round launches whose output rows fit the per-thread accumulators

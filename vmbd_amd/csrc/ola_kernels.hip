@@ -1293,13 +1293,23 @@ static void c2r_ola_launch(const Geo& g, const OlaDst& d, const float2* tw, hipS
     hipLaunchKernelGGL((k_rows_c2r_ola<TW, false, false, WGS>), grid, dim3(WGS), shm, st, g, d, tw);
 }
 
+// Output rows per workgroup of a summed-form launch of `rows` output rows
+// (all frames): three at 32 frames of 1024 rows, fewer when the grid would
+// not fill the GPU several times over (one 1024-row frame: one row per
+// workgroup, 1024 workgroups).
+void ola_fit_rows(OlaDst& d, long long rows) {
+  if (d.sumr > 1) d.sumr = (int)std::min<long long>(d.sumr, std::max<long long>(1, rows / 2048));
+}
+
 int ola_fits(const Geo& g, const OlaDst& d, int wgs) {
   return d.alias && (g.Tw / 16) * d.mc * ((g.csf1 + 1) / 2) <= wgs;
 }
 
 template <int TW>
-static hipError_t c2r_ola_t(const Geo& g, const OlaDst& d, const float2* tw, hipStream_t st, int wgs) {
+static hipError_t c2r_ola_t(const Geo& g, const OlaDst& d0, const float2* tw, hipStream_t st, int wgs) {
   const int frames = g.P / g.Pf;
+  OlaDst d = d0;
+  ola_fit_rows(d, (long long)d.O0 * frames);
   const size_t shm = ola_shared_bytes(g, d);
   ProfScope ps(d.win ? "rows_c2r_ola_win" : "rows_c2r_ola", st);
   if (d.sumr) {  // summed form: 256 threads, sumr output rows per workgroup
