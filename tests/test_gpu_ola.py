@@ -382,3 +382,4 @@ def test_separable_windows(geom, cuda):
     p_bad2d = OlaPlan(sx, sf, wb, device=cuda, separable=False)
     assert not p_bad.separable
     assert torch.equal(p_bad.adjoint_x(gt, ft), p_bad2d.adjoint_x(gt, ft))
+

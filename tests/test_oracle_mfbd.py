@@ -93,3 +93,17 @@ def test_rl_saturation_mask_and_clip():
     y2 = y.copy()
     y2[:4] = 7.0                       # other saturated values: same result
     np.testing.assert_array_equal(M.rl_step(x, f, y2, wa, sf), xn)
+
+
+def test_chimney_oracle_trajectory_fixture():
+    """The recorded FP64 trajectory of the reference's chimney loop (bars of
+    the GPU chimney test): the generator reproduces its first frame, and
+    the loop loses quality after its early peak (both transposes)."""
+    import json
+    import os
+    from tests.golden import make_chimney_oracle_psnr as G
+    rec = json.load(open(os.path.join(os.path.dirname(G.__file__), 'chimney_oracle_psnr.json')))
+    for v in ('fixed', 'k1'):
+        t = {int(k): x for k, x in rec[v].items()}
+        assert t[100] < t[10] < t[5], v
+    assert abs(G.trajectory('fixed', 1)['1'] - rec['fixed']['1']) <= 0.01

@@ -26,9 +26,11 @@ def frame_path(datapath, i, pattern='y_%04d.png'):
 
 
 def run(datapath, N=100, N0=20, sf=(20, 20), csf=(3, 3), overlap=0.5, f_alpha=0., f_beta=0.1, optiter=500,
-        tol=1e-10, f_solver='lbfgsb', f_iters=20, f_init='warm', capture=False, respath=None, verbose=False):
+        tol=1e-10, f_solver='lbfgsb', f_iters=20, f_init='warm', capture=False, respath=None, verbose=False,
+        snapshots=()):
     """Process frames 1..N; returns (latent image cropped by floor(sf/2) and
-    divided by its clip level, PSFs, seconds)."""
+    divided by its clip level, PSFs, seconds).  ``snapshots``: frame numbers
+    whose latent image (same crop and scale) is kept in ``run.snapshots``."""
     sf = np.broadcast_to(np.asarray(sf, dtype=int), (2,))
     load = lambda i: imread(frame_path(datapath, i))
     y0 = initial_latent([load(i) for i in range(1, N0)], N0)
@@ -39,9 +41,13 @@ def run(datapath, N=100, N0=20, sf=(20, 20), csf=(3, 3), overlap=0.5, f_alpha=0.
         os.makedirs(respath, exist_ok=True)
     t0 = time.perf_counter()
     xi = fs = None
+    run.snapshots = {}
     for i in range(1, N + 1):
         y = load(i)
         x, f = m.step(y)
+        if i in snapshots:
+            xh = x.cpu().numpy()
+            run.snapshots[i] = xh[sf2[0]:-sf2[0], sf2[1]:-sf2[1]] / float(m.xmax.item())
         if respath or i == N:
             x_max = float(m.xmax.item())
             xh = x.cpu().numpy()
