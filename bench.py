@@ -84,9 +84,12 @@ def kernel_models(plan, B):
         'rows_r2c': dict(bytes=B * (sy0 * sy1 * 4 + plan.csf[1] * NB * nv * 8),
                          flops=B * plan.csf[1] * half(nv) * rf),
         'cols_fwd': dict(bytes=B * P * NB * (sw0 + sf0 + ext0) * 8, flops=B * P * NB * (3 * cf + 6 * Th)),
-        'rows_c2r_ola': dict(bytes=B * (P * ext0 * NB * 8 + sy0 * sy1 * 4), flops=B * P * ext0 / 2 * rf),
+        # summed overlap-add: one inverse row transform per output row and
+        # pair of patch columns (the covering rows added before it)
+        'rows_c2r_ola': dict(bytes=B * (P * ext0 * NB * 8 + sy0 * sy1 * 4), flops=B * sy0 * half(plan.csf[1]) * rf),
         'cols_adj2': dict(bytes=B * P * NB * (ext0 + 2 * sw0 + 2 * sf0) * 8, flops=B * P * NB * (5 * cf + 12 * Th)),
-        'rows_c2r_ola_win': dict(bytes=B * (P * sw0 * NB * 8 + H * W * 4), flops=B * P * sw0 / 2 * rf),
+        'rows_c2r_ola_win': dict(bytes=B * (P * sw0 * NB * 8 + H * W * 4),
+                                 flops=B * (H * half(plan.csf[1]) if plan.separable else P * sw0 / 2) * rf),
         'rows_c2r_stack': dict(bytes=B * (P * sf0 * NB * 8 + P * sf0 * sf1 * 4), flops=B * P * half(sf0) * rf),
     }
     return m
