@@ -140,10 +140,21 @@ __device__ __forceinline__ void rows_r2c_body(Geo g, RowSrc s, float2* __restric
     }
     wave_sync();
     float2 v[16];
+    // columns at or past chi are zero: a job whose rows end within 3*TPT
+    // columns (PSF rows) takes a transform whose first pass knows the other
+    // slots are zero (uniform per job; not at 512 points, where the second
+    // transform takes the kernel past 128 VGPRs)
+    if (TW <= 256 && s.chi <= 3 * TPT) {
 #pragma unroll
-    for (int r = 0; r < 16; ++r) v[r] = sl[lds_pad(t + TPT * r)];
-    wave_sync();
-    fft<TW, -1>(v, sl, t, T, tw);
+      for (int r = 0; r < 3; ++r) v[r] = sl[lds_pad(t + TPT * r)];
+      wave_sync();
+      fft<TW, -1, (TW <= 256 ? 3 : 16)>(v, sl, t, T, tw);
+    } else {
+#pragma unroll
+      for (int r = 0; r < 16; ++r) v[r] = sl[lds_pad(t + TPT * r)];
+      wave_sync();
+      fft<TW, -1>(v, sl, t, T, tw);
+    }
     // natural-order spectrum of z = a + i b into the slice, then split
 #pragma unroll
     for (int mm = 0; mm < 16; ++mm) sl[lds_pad(t + TPT * mm)] = v[mm];
