@@ -76,18 +76,27 @@ def kernel_models(plan, B):
     sy0, sy1 = plan.sy
     rf, cf = fft_flops(Tw), fft_flops(Th)
     half = lambda r: (r + 1) // 2
+    # separable windows: image rows shared by the patches of a patch column,
+    # (csf0-1)*hop0 + sw0 rows per patch column (per-row patch factors applied
+    # by the column pass)
+    sx0 = (plan.csf[0] - 1) * plan.nhop[0] + sw0
+    xrows = plan.csf[1] * sx0 if plan.separable else P * sw0
     m = {
         'rows_r2c_multi': dict(bytes=B * (H * W * 4 + P * NB * sw0 * 8 + P * sf0 * sf1 * 4 + P * NB * sf0 * 8),
                                flops=B * P * (half(sw0) + half(sf0)) * rf),
+        'rows_r2c_x': dict(bytes=B * (H * W * 4 + plan.csf[1] * NB * sx0 * 8),
+                           flops=B * plan.csf[1] * half(sx0) * rf),
+        'rows_r2c_psf': dict(bytes=B * (P * sf0 * sf1 * 4 + P * NB * sf0 * 8), flops=B * P * half(sf0) * rf),
         # residual rows shared by the patches of a patch column: one transform
         # per (patch column, virtual row pair) over (csf0-1)*hop0 + ext0 rows
         'rows_r2c': dict(bytes=B * (sy0 * sy1 * 4 + plan.csf[1] * NB * nv * 8),
                          flops=B * plan.csf[1] * half(nv) * rf),
-        'cols_fwd': dict(bytes=B * P * NB * (sw0 + sf0 + ext0) * 8, flops=B * P * NB * (3 * cf + 6 * Th)),
+        'cols_fwd': dict(bytes=B * NB * (xrows + P * (sf0 + ext0)) * 8, flops=B * P * NB * (3 * cf + 6 * Th)),
         # summed overlap-add: one inverse row transform per output row and
         # pair of patch columns (the covering rows added before it)
         'rows_c2r_ola': dict(bytes=B * (P * ext0 * NB * 8 + sy0 * sy1 * 4), flops=B * sy0 * half(plan.csf[1]) * rf),
-        'cols_adj2': dict(bytes=B * P * NB * (ext0 + 2 * sw0 + 2 * sf0) * 8, flops=B * P * NB * (5 * cf + 12 * Th)),
+        'cols_adj2': dict(bytes=B * NB * (xrows + P * (ext0 + sw0 + 2 * sf0)) * 8,
+                          flops=B * P * NB * (5 * cf + 12 * Th)),
         'rows_c2r_ola_win': dict(bytes=B * (P * sw0 * NB * 8 + H * W * 4),
                                  flops=B * (H * half(plan.csf[1]) if plan.separable else P * sw0 / 2) * rf),
         'rows_c2r_stack': dict(bytes=B * (P * sf0 * NB * 8 + P * sf0 * sf1 * 4), flops=B * P * half(sf0) * rf),

@@ -35,16 +35,20 @@ def test_mixed_batch_vs_oracle_and_single_plans(cuda, grouped):
     xs, fs, gs = _frames(cuda)
     mb = MixedBatch(SPECS, grouped=grouped)
     assert len(mb.groups) == 3
-    # grouped over three streams: one batch per tile, seven launches each
-    assert mb.launches == (7 * len({G['plan'].tile for G in mb.groups}) if grouped else 21)
+    # separable windows give the image rows a pass of their own (eight
+    # launches per step instead of seven)
+    sep = int(mb.groups[0]['plan'].separable)
+    assert all(G['plan'].separable == bool(sep) for G in mb.groups)
+    # grouped over three streams: one batch per tile, 7 + sep launches each
+    assert mb.launches == ((7 + sep) * len({G['plan'].tile for G in mb.groups}) if grouped else 3 * (7 + sep))
     if grouped:
         # on one stream one batch of all groups: one launch per pass and
-        # distinct tile (width for the five row, overlap-add and stack
+        # distinct tile (width for the 5 + sep row, overlap-add and stack
         # passes, height for the two column passes)
         m1 = MixedBatch(SPECS, grouped=True, streams=1)
         n_tw = len({G['plan'].tile[1] for G in m1.groups})
         n_th = len({G['plan'].tile[0] for G in m1.groups})
-        assert m1.launches == 5 * n_tw + 2 * n_th
+        assert m1.launches == (5 + sep) * n_tw + 2 * n_th
     mb.load(xs, fs, gs)
     mb.step()
     torch.cuda.synchronize()

@@ -14,10 +14,13 @@ launch forms:
   launch per pass and distinct tile (the row, overlap-add and stack passes
   are specialised to the tile width, the column passes to its height).  With
   ``streams=1`` one batch holds every group (64 frames of six geometries on
-  three tiles: 19 launches per step instead of 42); with ``streams=k > 1``
+  three tiles: 22 launches per step instead of 48); with ``streams=k > 1``
   the groups are split by tile into one batch per tile and ``capture()``
-  forks those batches over k streams (21 launches on three streams).
-* ``grouped=False``: the groups' own launches, seven per group; with
+  forks those batches over k streams (24 launches on three streams).
+  Groups with separable windows (the default windows) transform their image
+  rows once per patch column in a row pass of their own.
+* ``grouped=False``: the groups' own launches, seven per group (eight with
+  separable windows); with
   ``streams=k > 1`` (default 3) ``capture()`` forks the groups over k
   streams inside the graph (largest groups first, each to the stream with
   the fewest pixels so far) so that one group's launches fill the tail of
@@ -135,7 +138,7 @@ class MixedBatch:
         """Kernel launches per step."""
         if self.grouped:
             return sum(int(_lib.lib().vmbd_batch_launches(vb)) for vb in self._batches().values())
-        return 7 * len(self.groups)
+        return sum(7 + int(G['plan'].separable) for G in self.groups)
 
     def _batch_step(self, vb):
         _lib.check(_lib.lib().vmbd_batch_step(vb, _lib.stream_ptr()), 'vmbd_batch_step')

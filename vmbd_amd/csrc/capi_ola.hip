@@ -47,6 +47,16 @@ struct vmbd_plan {
   float* fac_colw = nullptr;
   int* fac_zrow = nullptr;
   int fac_bcol[2] = {-1, -1}, fac_bn[2] = {0, 0}, fac_bc[2] = {0, 0};
+  // image rows shared by the patches of a patch column (separable windows):
+  // the row scale of each patch row [csf0][sw0], the column weights of each
+  // patch column and row type [csf1][2][sw1], the row type of each canvas
+  // row (1: every covering window at the floor) and 1/(covering patch rows)
+  // of the rows with real weights (0 on the others) [S0]
+  int xs = 0;
+  float* fac_rsc = nullptr;
+  float* fac_cw = nullptr;
+  float* fac_einv = nullptr;
+  int* fac_czr = nullptr;
 };
 
 namespace {
@@ -90,6 +100,10 @@ void plan_free(vmbd_plan* p) {
   (void)hipFree(p->fac_b);
   (void)hipFree(p->fac_colw);
   (void)hipFree(p->fac_zrow);
+  (void)hipFree(p->fac_rsc);
+  (void)hipFree(p->fac_cw);
+  (void)hipFree(p->fac_einv);
+  (void)hipFree(p->fac_czr);
   delete p;
 }
 
@@ -200,6 +214,52 @@ ColIn shared_in(const float2* ws, const SharedRows& r) {
   return ColIn{ws, r.s.rlo, r.nr, 0, row_pitch(r.nv), 1};
 }
 
+// Image rows of a call: one transform per (patch, tile row), windowed by the
+// patch's window; with separable windows one per (frame, patch column,
+// canvas row) over the S0 = (csf0-1)*hop0 + sw0 canvas rows, weighted by the
+// patch column's column weights of the row's type, and the column pass
+// scales tile row r of patch row m by rsc[m][r] (and adds the floor
+// columns' own terms): the rows of one image row are the same for every
+// patch row that covers it, up to that scale.
+struct XRows {
+  Geo g;     // row-pass geometry
+  RowSrc s;
+  ColIn in;  // the column pass's operand
+};
+
+XRows x_rows(const vmbd_plan* p, const Geo& g, const float* x) {
+  XRows r;
+  r.g = g;
+  r.s = src_image(p, x);
+  if (!p->xs) {
+    r.in = raw_in(p->wsX, r.s);
+    return r;
+  }
+  const int S0 = (g.csf0 - 1) * g.hop0 + g.sw0;
+  r.s.rlo = 0;
+  r.s.rhi = S0;
+  r.s.win = nullptr;
+  r.s.win_idx = nullptr;
+  r.s.cw = p->fac_cw;
+  r.s.czr = p->fac_czr;
+  r.g.csf0 = 1;
+  r.g.Pf = g.csf1;
+  r.g.P = (g.P / g.Pf) * g.csf1;
+  r.in = ColIn{p->wsX, 0, g.sw0, 0, row_pitch(S0), 1};
+  r.in.rsc = p->fac_rsc;
+  r.in.ex = x;
+  r.in.exf = (long long)g.H * g.W;
+  r.in.exld = g.W;
+  r.in.exH = g.H;
+  r.in.einv = p->fac_einv;
+  for (int e = 0; e < 2; ++e) {
+    r.in.ecol[e] = p->fac_bcol[e];
+    r.in.en[e] = p->fac_bn[e];
+    r.in.ec[e] = p->fac_bc[e];
+  }
+  return r;
+}
+
 float inv_scale(const Geo& g) { return 1.0f / (float)((double)g.Th * (double)g.Tw); }
 
 OlaDst dst_forward(const vmbd_plan* p, float* y) {
@@ -255,6 +315,19 @@ OlaDst dst_adjoint_x(const vmbd_plan* p, float* xbar) {
   }
   ola_layout(g, d);
   return d;
+}
+
+// row transforms of x and f: one launch over both (one geometry), or, with
+// the image rows shared per patch column, one launch each
+hipError_t launch_xf_rows(const vmbd_plan* p, const Geo& g, const XRows& xr, const RowSrc& sf, hipStream_t st) {
+  if (!p->xs) {
+    RowSrc srcs[2] = {xr.s, sf};
+    float2* outs[2] = {p->wsX, p->wsF};
+    return launch_rows_r2c_multi(g, srcs, outs, 2, p->tw, st);
+  }
+  hipError_t e = launch_rows_r2c(xr.g, xr.s, p->wsX, p->tw, st, "rows_r2c_x");
+  if (e == hipSuccess) e = launch_rows_r2c(g, sf, p->wsF, p->tw, st, "rows_r2c_psf");
+  return e;
 }
 
 #define VMBD_LAUNCH(expr)                                            \
@@ -545,21 +618,67 @@ int vmbd_plan_set_window_factors(vmbd_plan* p, const float* a, const float* b, f
     colw[v] = B > 0.0 ? (float)(1.0 / B) : 0.f;
     colw[(size_t)W + v] = cn ? (float)(1.0 / cn) : 0.f;
   }
+  // image rows shared per patch column: tile row r of patch row m is
+  // rsc[m][r] times the canvas row u = m*hop0 + r weighted by the column
+  // weights of u's type; wimg = A(u) B(v) away from the floor, 1/count on it
+  std::vector<float> rsc((size_t)g.csf0 * sw0), cw((size_t)g.csf1 * 2 * sw1), einv(S0);
+  std::vector<int> czr(S0);
+  std::vector<double> A(S0, 0.0);
+  std::vector<int> M(S0, 0);
+  for (int u = 0; u < S0; ++u) {
+    for (int m = 0; m < g.csf0; ++m) {
+      const int r = u - m * hop0;
+      if (r < 0 || r >= sw0) continue;
+      A[u] += a[r];
+      ++M[u];
+    }
+    czr[u] = floor_line(a, g.csf0, hop0, sw0, u);
+    einv[u] = (czr[u] || !M[u]) ? 0.f : (float)(1.0 / M[u]);
+  }
+  for (int m = 0; m < g.csf0; ++m)
+    for (int r = 0; r < sw0; ++r) {
+      const int u = m * hop0 + r;
+      rsc[(size_t)m * sw0 + r] = czr[u] ? (float)(1.0 / M[u]) : (A[u] > 0.0 ? (float)(a[r] / A[u]) : 0.f);
+    }
+  for (int n = 0; n < g.csf1; ++n)
+    for (int c = 0; c < sw1; ++c) {
+      const int v = n * hop1 + c;
+      double B = 0.0;
+      int cn = 0;
+      for (int n2 = 0; n2 < g.csf1; ++n2) {
+        const int c2 = v - n2 * hop1;
+        if (c2 < 0 || c2 >= sw1) continue;
+        B += b[c2];
+        ++cn;
+      }
+      cw[((size_t)n * 2 + 0) * sw1 + c] = B > 0.0 ? (float)(b[c] / B) : 0.f;
+      cw[((size_t)n * 2 + 1) * sw1 + c] = cn ? (float)(1.0 / cn) : 0.f;
+    }
   size_t tot = 0;
   if (!(dev_alloc(&p->fac_a, sw0, tot) && dev_alloc(&p->fac_b, sw1, tot) && dev_alloc(&p->fac_colw, colw.size(), tot) &&
-        dev_alloc(&p->fac_zrow, zrow.size(), tot))) {
+        dev_alloc(&p->fac_zrow, zrow.size(), tot) && dev_alloc(&p->fac_rsc, rsc.size(), tot) &&
+        dev_alloc(&p->fac_cw, cw.size(), tot) && dev_alloc(&p->fac_einv, einv.size(), tot) &&
+        dev_alloc(&p->fac_czr, czr.size(), tot))) {
     (void)hipFree(p->fac_a);
     (void)hipFree(p->fac_b);
     (void)hipFree(p->fac_colw);
     (void)hipFree(p->fac_zrow);
-    p->fac_a = p->fac_b = p->fac_colw = nullptr;
-    p->fac_zrow = nullptr;
+    (void)hipFree(p->fac_rsc);
+    (void)hipFree(p->fac_cw);
+    (void)hipFree(p->fac_einv);
+    (void)hipFree(p->fac_czr);
+    p->fac_a = p->fac_b = p->fac_colw = p->fac_rsc = p->fac_cw = p->fac_einv = nullptr;
+    p->fac_zrow = p->fac_czr = nullptr;
     return vmbd_fail(VMBD_E_NOMEM, "vmbd_plan_set_window_factors: hipMalloc failed");
   }
   e = hipMemcpy(p->fac_a, a, sw0 * sizeof(float), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(p->fac_b, b, sw1 * sizeof(float), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(p->fac_colw, colw.data(), colw.size() * sizeof(float), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(p->fac_zrow, zrow.data(), zrow.size() * sizeof(int), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(p->fac_rsc, rsc.data(), rsc.size() * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(p->fac_cw, cw.data(), cw.size() * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(p->fac_einv, einv.data(), einv.size() * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(p->fac_czr, czr.data(), czr.size() * sizeof(int), hipMemcpyHostToDevice);
   if (e != hipSuccess) return vmbd_fail_hip("vmbd_plan_set_window_factors: hipMemcpy", e);
   for (int e2 = 0; e2 < 2; ++e2) {
     p->fac_bcol[e2] = bcol[e2];
@@ -573,11 +692,17 @@ int vmbd_plan_set_window_factors(vmbd_plan* p, const float* a, const float* b, f
     (void)hipFree(p->fac_b);
     (void)hipFree(p->fac_colw);
     (void)hipFree(p->fac_zrow);
-    p->fac_a = p->fac_b = p->fac_colw = nullptr;
-    p->fac_zrow = nullptr;
+    (void)hipFree(p->fac_rsc);
+    (void)hipFree(p->fac_cw);
+    (void)hipFree(p->fac_einv);
+    (void)hipFree(p->fac_czr);
+    p->fac_a = p->fac_b = p->fac_colw = p->fac_rsc = p->fac_cw = p->fac_einv = nullptr;
+    p->fac_zrow = p->fac_czr = nullptr;
     p->bytes -= tot;
     return vmbd_fail(VMBD_E_UNSUP, "vmbd_plan_set_window_factors: summed x-adjoint does not fit the LDS");
   }
+  p->xs = 1;          // image rows shared per patch column from now on
+  p->rows_frames = 0;  // the workspaces hold no rows of the new layout
   return VMBD_OK;
 }
 
@@ -609,13 +734,13 @@ int vmbd_ola_forward(vmbd_plan* p, int32_t nframes, const float* x, const float*
   hipStream_t st = (hipStream_t)stream;
   const Geo g = call_geo(p, nframes);
   p->rows_frames = 0;
-  RowSrc srcs[2] = {src_image(p, x), src_psf(p, f)};
-  float2* outs[2] = {p->wsX, p->wsF};
-  VMBD_LAUNCH(launch_rows_r2c_multi(g, srcs, outs, 2, p->tw, st));
+  const XRows xr = x_rows(p, g, x);
+  const RowSrc sf = src_psf(p, f);
+  VMBD_LAUNCH(launch_xf_rows(p, g, xr, sf, st));
   ColOut o1{p->Y1, g.ext0}, o2{nullptr, 0};
   ColIn none{nullptr, 0, 0, 0};
-  VMBD_LAUNCH(launch_cols(g, COL_FWD, raw_in(p->wsX, srcs[0]), raw_in(p->wsF, srcs[1]), none, o1, o2, nullptr,
-                          p->tw, st, p->tune.col_threads));
+  VMBD_LAUNCH(launch_cols(g, COL_FWD, xr.in, raw_in(p->wsF, sf), none, o1, o2, nullptr, p->tw, st,
+                          p->tune.col_threads));
   VMBD_LAUNCH(launch_rows_c2r_ola(g, dst_forward(p, y), p->tw, st, p->tune.ola_threads));
   p->rows_frames = nframes;
   return VMBD_OK;
@@ -632,7 +757,7 @@ int vmbd_ola_adjoint_x(vmbd_plan* p, int32_t nframes, const float* gres, const f
   const SharedRows sr = residual_rows(p, g, gres, exact);
   const RowSrc sf = src_psf(p, f);
   VMBD_LAUNCH(launch_rows_r2c(sr.g, sr.s, p->wsG, p->tw, st));
-  VMBD_LAUNCH(launch_rows_r2c(g, sf, p->wsF, p->tw, st));
+  VMBD_LAUNCH(launch_rows_r2c(g, sf, p->wsF, p->tw, st, "rows_r2c_psf"));
   ColOut o1{p->Y1, g.sw0}, o2{nullptr, 0};
   ColIn none{};
   VMBD_LAUNCH(launch_cols(g, COL_ADJ1, shared_in(p->wsG, sr), raw_in(p->wsF, sf), none, o1, o2, nullptr, p->tw, st, p->tune.col_threads));
@@ -649,12 +774,12 @@ int vmbd_ola_adjoint_f(vmbd_plan* p, int32_t nframes, const float* gres, const f
   const Geo g = call_geo(p, nframes);
   p->rows_frames = 0;
   const SharedRows sr = residual_rows(p, g, gres, exact);
-  const RowSrc sx = src_image(p, x);
+  const XRows xr = x_rows(p, g, x);
   VMBD_LAUNCH(launch_rows_r2c(sr.g, sr.s, p->wsG, p->tw, st));
-  VMBD_LAUNCH(launch_rows_r2c(g, sx, p->wsX, p->tw, st));
+  VMBD_LAUNCH(launch_rows_r2c(xr.g, xr.s, p->wsX, p->tw, st, "rows_r2c_x"));
   ColOut o1{p->Y2, g.sf0}, o2{nullptr, 0};
   ColIn none{};
-  VMBD_LAUNCH(launch_cols(g, COL_ADJ1, shared_in(p->wsG, sr), raw_in(p->wsX, sx), none, o1, o2, nullptr, p->tw, st, p->tune.col_threads));
+  VMBD_LAUNCH(launch_cols(g, COL_ADJ1, shared_in(p->wsG, sr), xr.in, none, o1, o2, nullptr, p->tw, st, p->tune.col_threads));
   VMBD_LAUNCH(launch_rows_c2r_stack(g, p->Y2, g.sf0, fbar, g.sf0, g.sf1, inv_scale(g), p->tw, st));
   return VMBD_OK;
 }
@@ -672,18 +797,17 @@ int vmbd_ola_adjoint_xf(vmbd_plan* p, int32_t nframes, const float* gres, const 
   hipStream_t st = (hipStream_t)stream;
   const Geo g = call_geo(p, nframes);
   const SharedRows sr = residual_rows(p, g, gres, exact);
-  RowSrc sx = src_image(p, x), sf = src_psf(p, f);
+  const XRows xr = x_rows(p, g, x);
+  const RowSrc sf = src_psf(p, f);
   VMBD_LAUNCH(launch_rows_r2c(sr.g, sr.s, p->wsG, p->tw, st));
   if (!reuse) {
     p->rows_frames = 0;
-    RowSrc srcs[2] = {sx, sf};
-    float2* outs[2] = {p->wsX, p->wsF};
-    VMBD_LAUNCH(launch_rows_r2c_multi(g, srcs, outs, 2, p->tw, st));
+    VMBD_LAUNCH(launch_xf_rows(p, g, xr, sf, st));
     p->rows_frames = nframes;  // wsX/wsF now hold the rows of this x and f
   }
   ColOut o1{p->Y1, g.sw0}, o2{p->Y2, g.sf0};
-  VMBD_LAUNCH(launch_cols(g, COL_ADJ2, shared_in(p->wsG, sr), raw_in(p->wsF, sf), raw_in(p->wsX, sx), o1, o2, nullptr,
-                          p->tw, st, p->tune.col_threads));
+  VMBD_LAUNCH(launch_cols(g, COL_ADJ2, shared_in(p->wsG, sr), raw_in(p->wsF, sf), xr.in, o1, o2, nullptr, p->tw, st,
+                          p->tune.col_threads));
   VMBD_LAUNCH(launch_rows_c2r_ola(g, dst_adjoint_x(p, xbar), p->tw, st, p->tune.ola_threads));
   VMBD_LAUNCH(launch_rows_c2r_stack(g, p->Y2, g.sf0, fbar, g.sf0, g.sf1, inv_scale(g), p->tw, st));
   return VMBD_OK;
@@ -729,12 +853,19 @@ int vmbd_spectra(vmbd_plan* p, int32_t nframes, int32_t kind, const float* z, vo
   hipStream_t st = (hipStream_t)stream;
   const Geo g = call_geo(p, nframes);
   p->rows_frames = 0;
-  RowSrc s = kind == VMBD_CACHE_PSF ? src_psf(p, z) : src_image(p, z);
-  float2* ws = kind == VMBD_CACHE_PSF ? p->wsF : p->wsX;
-  VMBD_LAUNCH(launch_rows_r2c(g, s, ws, p->tw, st));
+  ColIn in;
+  if (kind == VMBD_CACHE_PSF) {
+    const RowSrc s = src_psf(p, z);
+    VMBD_LAUNCH(launch_rows_r2c(g, s, p->wsF, p->tw, st, "rows_r2c_psf"));
+    in = raw_in(p->wsF, s);
+  } else {
+    const XRows xr = x_rows(p, g, z);
+    VMBD_LAUNCH(launch_rows_r2c(xr.g, xr.s, p->wsX, p->tw, st, "rows_r2c_x"));
+    in = xr.in;
+  }
   ColOut none_o{nullptr, 0};
   ColIn none{nullptr, 0, 0, 0};
-  VMBD_LAUNCH(launch_cols(g, COL_SPEC, raw_in(ws, s), none, none, none_o, none_o, (float2*)spec, p->tw, st, p->tune.col_threads));
+  VMBD_LAUNCH(launch_cols(g, COL_SPEC, in, none, none, none_o, none_o, (float2*)spec, p->tw, st, p->tune.col_threads));
   return VMBD_OK;
 }
 
@@ -749,12 +880,19 @@ int vmbd_ola_forward_cached_ex(vmbd_plan* p, int32_t nframes, int32_t kind, cons
   const Geo g = call_geo(p, nframes);
   p->rows_frames = 0;
   // the free operand is the image for a PSF cache and the PSFs for an image cache
-  RowSrc s = kind == VMBD_CACHE_PSF ? src_image(p, u) : src_psf(p, u);
-  float2* ws = kind == VMBD_CACHE_PSF ? p->wsX : p->wsF;
-  VMBD_LAUNCH(launch_rows_r2c(g, s, ws, p->tw, st));
+  ColIn in;
+  if (kind == VMBD_CACHE_PSF) {
+    const XRows xr = x_rows(p, g, u);
+    VMBD_LAUNCH(launch_rows_r2c(xr.g, xr.s, p->wsX, p->tw, st, "rows_r2c_x"));
+    in = xr.in;
+  } else {
+    const RowSrc s = src_psf(p, u);
+    VMBD_LAUNCH(launch_rows_r2c(g, s, p->wsF, p->tw, st, "rows_r2c_psf"));
+    in = raw_in(p->wsF, s);
+  }
   ColOut o1{p->Y1, g.ext0}, o2{nullptr, 0};
   ColIn none{};
-  VMBD_LAUNCH(launch_cols(g, COL_FWD, raw_in(ws, s), spec_in(spec, g.Th), none, o1, o2, nullptr, p->tw, st, p->tune.col_threads));
+  VMBD_LAUNCH(launch_cols(g, COL_FWD, in, spec_in(spec, g.Th), none, o1, o2, nullptr, p->tw, st, p->tune.col_threads));
   OlaDst d = dst_forward(p, y);
   d.sub = sub;
   d.mul = mul;
@@ -1015,17 +1153,30 @@ BatchGroup batch_group(vmbd_plan* p, int nframes, const float* x, const float* f
   const Geo g = call_geo(p, nframes);
   B.g = g;
   B.tw = p->tw;
-  const RowSrc sx = src_image(p, x), sf = src_psf(p, f);
+  const XRows xr = x_rows(p, g, x);
+  const RowSrc sf = src_psf(p, f);
   const int TPW = WG / (g.Tw / 16);
   auto blocks = [&](const RowSrc& s) { return ((s.rhi - s.rlo + 1) / 2 + TPW - 1) / TPW; };
-  B.rows_fwd.n = 2;
-  B.rows_fwd.s[0] = sx;
-  B.rows_fwd.s[1] = sf;
-  B.rows_fwd.out[0] = p->wsX;
-  B.rows_fwd.out[1] = p->wsF;
-  B.rows_fwd.blk_end[0] = blocks(sx);
-  B.rows_fwd.blk_end[1] = B.rows_fwd.blk_end[0] + blocks(sf);
-  B.cf_a = raw_in(p->wsX, sx);
+  if (p->xs) {  // the image rows in a pass of their own (their geometry)
+    B.rows_fwd.n = 1;
+    B.rows_fwd.s[0] = sf;
+    B.rows_fwd.out[0] = p->wsF;
+    B.rows_fwd.blk_end[0] = blocks(sf);
+    B.gx = xr.g;
+    B.rows_x.n = 1;
+    B.rows_x.s[0] = xr.s;
+    B.rows_x.out[0] = p->wsX;
+    B.rows_x.blk_end[0] = blocks(xr.s);
+  } else {
+    B.rows_fwd.n = 2;
+    B.rows_fwd.s[0] = xr.s;
+    B.rows_fwd.s[1] = sf;
+    B.rows_fwd.out[0] = p->wsX;
+    B.rows_fwd.out[1] = p->wsF;
+    B.rows_fwd.blk_end[0] = blocks(xr.s);
+    B.rows_fwd.blk_end[1] = B.rows_fwd.blk_end[0] + blocks(sf);
+  }
+  B.cf_a = xr.in;
   B.cf_b = raw_in(p->wsF, sf);
   B.cf_o = ColOut{p->Y1, g.ext0};
   B.ola_fwd = dst_forward(p, y);
@@ -1038,7 +1189,7 @@ BatchGroup batch_group(vmbd_plan* p, int nframes, const float* x, const float* f
   B.rows_res.blk_end[0] = blocks(sr.s);
   B.ca_a = shared_in(p->wsG, sr);
   B.ca_b = raw_in(p->wsF, sf);
-  B.ca_c = raw_in(p->wsX, sx);
+  B.ca_c = xr.in;
   B.ca_o1 = ColOut{p->Y1, g.sw0};
   B.ca_o2 = ColOut{p->Y2, g.sf0};
   B.ola_adj = dst_adjoint_x(p, xbar);
@@ -1102,7 +1253,7 @@ vmbd_batch* vmbd_batch_create(vmbd_plan* const* plans, int32_t n, const int32_t*
       l.tile = t;
       int blk = 0;
       for (int i = 0; i < n; ++i) {
-        if (batch_tile(host[i], pass) != t) continue;
+        if (batch_tile(host[i], pass) != t || batch_blocks(host[i], pass) == 0) continue;
         l.L.gidx[l.L.ng] = i;
         l.L.start[l.L.ng] = blk;
         ++l.L.ng;
@@ -1110,6 +1261,7 @@ vmbd_batch* vmbd_batch_create(vmbd_plan* const* plans, int32_t n, const int32_t*
         l.shm = std::max(l.shm, batch_shared_bytes(host[i], pass));
       }
       l.L.start[l.L.ng] = blk;
+      if (l.L.ng == 0) continue;  // no group of this tile has work in the pass
       l.wgs = (pass == BP_OLA_FWD || pass == BP_OLA_ADJ) ? batch_ola_threads(host.data(), l.L, pass) : WG;
       b->launches.push_back(l);
     }

@@ -22,10 +22,11 @@ struct cmax {
 __device__ __forceinline__ void ld4(float (&d)[4], const float* p) { __builtin_memcpy(d, p, 16); }
 
 // NZ < 16: only slots r < NZ can hold stored rows (rlo + nr <= NZ*TPT); the
-// rest are zero without a load
-template <int TH, int NZ = 16>
+// rest are zero without a load.  XS: the operand may be shared image rows
+// with patch-row factors (in.rsc); only those call sites carry that path.
+template <int TH, int NZ = 16, bool XS = false>
 __device__ __forceinline__ void col_load(float2 (&v)[16], const ColIn& in, int p, int c, int NB, int t, bool active,
-                                         int grp, int row0) {
+                                         int grp, int row0, const ColCtx& cx) {
   constexpr int TPT = TH / 16;
   if constexpr (NZ < 16) {
 #pragma unroll
@@ -35,6 +36,61 @@ __device__ __forceinline__ void col_load(float2 (&v)[16], const ColIn& in, int p
     const float2* col = in.p + ((size_t)p * NB + c) * TH;
 #pragma unroll
     for (int r = 0; r < 16; ++r) v[r] = active ? col[t + TPT * r] : make_float2(0.f, 0.f);
+  } else if (XS && in.rsc) {
+    // shared image rows, each scaled by its patch row's factor.  Row q of
+    // the operand sits in lane q % 64 of table register q / 64 (one coalesced
+    // load per 64 rows) and reaches the lanes that hold it in a transform by
+    // a cross-lane permute: the loads of a column issue together.
+    constexpr int K = (TPT * NZ + 63) / 64;
+    const int lane = threadIdx.x & 63;
+    const float2* col = in.p + ((size_t)grp * NB + c) * in.pitch + row0;
+    const float* rs = in.rsc + (size_t)cx.pm * in.nr;
+    float tab[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int q = 64 * k + lane - in.rlo;
+      tab[k] = (q >= 0 && q < in.nr) ? rs[q] : 0.f;
+    }
+#pragma unroll
+    for (int r = 0; r < NZ; ++r) {
+      const int rr = t + TPT * r - in.rlo;
+      v[r] = (active && rr >= 0 && rr < in.nr) ? col[rr] : make_float2(0.f, 0.f);
+    }
+#pragma unroll
+    for (int r = 0; r < NZ; ++r) {
+      const float sc = __shfl(tab[(TPT * r) / 64], (TPT * r) % 64 + t, 64);
+      v[r].x *= sc;
+      v[r].y *= sc;
+    }
+    // floor columns at the canvas edge: the pixel's own term, at its row's
+    // weight (no patch-row factor), as bin c of a unit impulse at column ec:
+    // e^{-2 pi i c ec / Tw}; the packed column holds bins 0 and Tw/2
+    // (factors 1 and (-1)^ec)
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      const int ecol = e ? in.ecol[1] : in.ecol[0];
+      const int en = e ? in.en[1] : in.en[0];
+      const int ec = e ? in.ec[1] : in.ec[0];
+      if (in.ex && ecol >= 0 && cx.pn == en) {
+        const float* xf = in.ex + (size_t)cx.pfr * in.exf + ecol;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          const int q = 64 * k + lane - in.rlo;
+          const int u = cx.pm * cx.hop0 + q;
+          tab[k] = (q >= 0 && q < in.nr && u < in.exH) ? xf[(size_t)u * in.exld] * in.einv[u] : 0.f;
+        }
+        const float2 E = c == 0 ? make_float2(1.f, (ec & 1) ? -1.f : 1.f)
+                                : cx.tw[((long long)c * ec * (TW_LEN / cx.Tw)) & (TW_LEN - 1)];
+#pragma unroll
+        for (int r = 0; r < NZ; ++r) {
+          const float dl = __shfl(tab[(TPT * r) / 64], (TPT * r) % 64 + t, 64);
+          if (active) {
+            v[r].x += dl * E.x;
+            v[r].y += dl * E.y;
+          }
+        }
+      }
+    }
   } else {
     const float2* col = in.shared ? in.p + ((size_t)grp * NB + c) * in.pitch + row0
                                   : in.p + ((size_t)p * NB + c) * in.pitch;

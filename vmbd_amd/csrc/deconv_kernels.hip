@@ -107,6 +107,7 @@ __global__ __launch_bounds__(WG, 1) void k_cols_div(Geo g, ColIn a0, ColIn a1, C
   const int pm = pk_ / g.csf1;
   const int grp = pfr * g.csf1 + (pk_ - pm * g.csf1);
   const int row0 = pm * g.hop0;
+  const ColCtx cx{pm, pk_ - pm * g.csf1, pfr, g.hop0, g.Tw, tw};
   float2* sl = sm + cc * SL;
 
   Twiddles<TH> T;
@@ -123,7 +124,7 @@ __global__ __launch_bounds__(WG, 1) void k_cols_div(Geo g, ColIn a0, ColIn a1, C
 #pragma unroll
   for (int j = 0; j < NA; ++j) {
     float2 v[16];
-    col_load<TH>(v, ins[j], p, c, NB, t, active, grp, row0);
+    col_load<TH>(v, ins[j], p, c, NB, t, active, grp, row0, cx);
     if (!ins[j].spec && wact) fft<TH, -1>(v, sl, t, T, tw);
     if (packed) {
 #pragma unroll

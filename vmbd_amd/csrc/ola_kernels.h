@@ -44,6 +44,11 @@ struct RowSrc {
   const float* win;
   const int* win_idx;
   int wsh0, wsh1;
+  // instead of win (when set): per-patch-column weights of one row type,
+  // cw[(k*2 + czr[i])*sw1 + c] for the source row i of tile row r (the image
+  // rows shared by a patch column, separable windows)
+  const float* cw;
+  const int* czr;
 };
 
 // Up to four row-transform jobs fused in one launch: blocks [0, blk_end[0])
@@ -71,6 +76,25 @@ struct ColIn {
   const float2* p;
   int rlo, nr, spec, pitch;
   int shared;
+  // shared image rows (separable windows): tile row r of patch (m, n) is
+  // rsc[m*nr + r] times its patch column's row; where the window is at the
+  // floor in a column covered by one patch column (ecol[e] >= 0: image
+  // column, en: its patch column, ec: its tile column) the row takes
+  // ex[frame][m*hop0 + r][ecol] * einv[m*hop0 + r] there (a constant times
+  // e^{-2 pi i k ec / Tw} in bin k), which the load adds
+  const float* rsc;
+  const float* ex;
+  long long exf;
+  int exld, exH;
+  const float* einv;
+  int ecol[2], en[2], ec[2];
+};
+
+// Where a column-pass workgroup is: patch row and column, frame, the hop,
+// the row bins, the twiddle table (the shared image rows' corrections)
+struct ColCtx {
+  int pm, pn, pfr, hop0, Tw;
+  const float2* tw;
 };
 
 // Column-pass output rows [0, nrows) in layout [P][nrows][NBP].
@@ -157,7 +181,9 @@ struct DivParams {
 };
 
 // Launchers (ola_kernels.hip).  Return hipError_t of the launch.
-hipError_t launch_rows_r2c(const Geo& g, const RowSrc& s, float2* out, const float2* tw, hipStream_t st);
+// name: the launch's label in the library's kernel timing (default rows_r2c)
+hipError_t launch_rows_r2c(const Geo& g, const RowSrc& s, float2* out, const float2* tw, hipStream_t st,
+                           const char* name = nullptr);
 // several row-transform jobs in one launch (n <= ROW_JOBS_MAX)
 hipError_t launch_rows_r2c_multi(const Geo& g, const RowSrc* s, float2* const* out, int n, const float2* tw,
                                  hipStream_t st);
@@ -187,7 +213,8 @@ constexpr int BATCH_MAX_GROUPS = 16;
 struct BatchGroup {
   Geo g;     // call geometry (P = frames * Pf)
   Geo gres;  // geometry of the shared residual rows
-  RowJobs rows_fwd, rows_res;
+  Geo gx;    // geometry of the image rows shared per patch column (rows_x.n > 0)
+  RowJobs rows_fwd, rows_res, rows_x;
   ColIn cf_a, cf_b;
   ColOut cf_o;
   ColIn ca_a, ca_b, ca_c;
@@ -199,7 +226,7 @@ struct BatchGroup {
   float st_scale;
   const float2* tw;
 };
-enum BatchPass { BP_ROWS_FWD = 0, BP_COLS_FWD, BP_OLA_FWD, BP_ROWS_RES, BP_COLS_ADJ, BP_OLA_ADJ, BP_STACK, BP_N };
+enum BatchPass { BP_ROWS_FWD = 0, BP_ROWS_X, BP_COLS_FWD, BP_OLA_FWD, BP_ROWS_RES, BP_COLS_ADJ, BP_OLA_ADJ, BP_STACK, BP_N };
 // One launch of a pass over ng groups of the same tile (Tw for the row, OLA
 // and stack passes, Th for the column passes): launch group gi is
 // groups[gidx[gi]] and owns blocks [start[gi], start[gi+1]); every start is a

@@ -87,8 +87,16 @@ __device__ __forceinline__ void rows_r2c_body(Geo g, RowSrc s, float2* __restric
   const bool ok1 = active && r0 + 1 < s.rhi && i1 >= 0 && i1 < s.srcH;
   const float* row0 = base + (size_t)(ok0 ? i0 : 0) * s.ld;
   const float* row1 = base + (size_t)(ok1 ? i1 : 0) * s.ld;
-  // window row r0 - wsh0, column c - wsh1 (only read inside [clo,chi))
+  // window rows of r0 and r0+1, column c - wsh1 (only read inside
+  // [clo,chi)): rows r0 - wsh0 and r0 + 1 - wsh0 of the patch's window, or
+  // the patch column's weights of each source row's type (cw)
+  const bool hw = wb || s.cw;
   const float* w0 = wb ? wb + ((long long)(r0 - s.wsh0) * g.sw1 - s.wsh1) : nullptr;
+  const float* w1 = wb ? w0 + g.sw1 : nullptr;
+  if (s.cw) {
+    w0 = s.cw + ((long long)(k * 2 + (ok0 ? s.czr[i0] : 0)) * g.sw1 - s.wsh1);
+    w1 = s.cw + ((long long)(k * 2 + (ok1 ? s.czr[i1] : 0)) * g.sw1 - s.wsh1);
+  }
 
   float2* sl = sm + tr * SL;
   float2 za[9], zb[9];
@@ -109,8 +117,8 @@ __device__ __forceinline__ void rows_r2c_body(Geo g, RowSrc s, float2* __restric
       for (int e = 0; e < 4; ++e) a[i][e] = b[i][e] = wa[i][e] = wb4[i][e] = 0.f;
       if (full[i] && ok0) ld4(a[i], row0 + j0);
       if (full[i] && ok1) ld4(b[i], row1 + j0);
-      if (wb && full[i] && ok0) ld4(wa[i], w0 + c0);
-      if (wb && full[i] && ok1) ld4(wb4[i], w0 + g.sw1 + c0);
+      if (hw && full[i] && ok0) ld4(wa[i], w0 + c0);
+      if (hw && full[i] && ok1) ld4(wb4[i], w1 + c0);
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -123,8 +131,8 @@ __device__ __forceinline__ void rows_r2c_body(Geo g, RowSrc s, float2* __restric
           const bool okc = c >= s.clo && c < s.chi && j >= 0 && j < s.srcW;
           if (ok0 && okc) a[i][e] = row0[j];
           if (ok1 && okc) b[i][e] = row1[j];
-          if (wb && ok0 && okc) wa[i][e] = w0[c];
-          if (wb && ok1 && okc) wb4[i][e] = w0[g.sw1 + c];
+          if (hw && ok0 && okc) wa[i][e] = w0[c];
+          if (hw && ok1 && okc) wb4[i][e] = w1[c];
         }
       }
     }
@@ -133,8 +141,8 @@ __device__ __forceinline__ void rows_r2c_body(Geo g, RowSrc s, float2* __restric
       const int c0 = 4 * (t + TPT * i);
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const float x0 = wb ? a[i][e] * wa[i][e] : a[i][e];
-        const float x1 = wb ? b[i][e] * wb4[i][e] : b[i][e];
+        const float x0 = hw ? a[i][e] * wa[i][e] : a[i][e];
+        const float x1 = hw ? b[i][e] * wb4[i][e] : b[i][e];
         sl[lds_pad(c0 + e)] = make_float2(x0, x1);
       }
     }
@@ -250,7 +258,7 @@ __global__ __launch_bounds__(WG) void k_rows_r2c(Geo g, RowJobs jobs, const floa
 
 template <int TW>
 static hipError_t rows_r2c_t(const Geo& g, const RowSrc* s, float2* const* out, int n, const float2* tw,
-                             hipStream_t st) {
+                             hipStream_t st, const char* name) {
   constexpr int TPW = WG / (TW / 16);
   RowJobs jobs{};
   jobs.n = n;
@@ -263,27 +271,33 @@ static hipError_t rows_r2c_t(const Geo& g, const RowSrc* s, float2* const* out, 
     jobs.blk_end[i] = blocks;
   }
   if (blocks <= 0 || g.P <= 0) return hipSuccess;
-  ProfScope ps(n == 1 ? "rows_r2c" : "rows_r2c_multi", st);
+  ProfScope ps(name ? name : n == 1 ? "rows_r2c" : "rows_r2c_multi", st);
   hipLaunchKernelGGL(k_rows_r2c<TW>, dim3((unsigned)(g.P * blocks)), dim3(WG), 0, st, g, jobs, tw);
   return hipGetLastError();
 }
 
-hipError_t launch_rows_r2c_multi(const Geo& g, const RowSrc* s, float2* const* out, int n, const float2* tw,
-                                 hipStream_t st) {
+static hipError_t rows_r2c_any(const Geo& g, const RowSrc* s, float2* const* out, int n, const float2* tw,
+                               hipStream_t st, const char* name) {
   if (n < 1 || n > ROW_JOBS_MAX) return hipErrorInvalidValue;
   switch (g.Tw) {
-    case 16: return rows_r2c_t<16>(g, s, out, n, tw, st);
-    case 32: return rows_r2c_t<32>(g, s, out, n, tw, st);
-    case 64: return rows_r2c_t<64>(g, s, out, n, tw, st);
-    case 128: return rows_r2c_t<128>(g, s, out, n, tw, st);
-    case 256: return rows_r2c_t<256>(g, s, out, n, tw, st);
-    case 512: return rows_r2c_t<512>(g, s, out, n, tw, st);
+    case 16: return rows_r2c_t<16>(g, s, out, n, tw, st, name);
+    case 32: return rows_r2c_t<32>(g, s, out, n, tw, st, name);
+    case 64: return rows_r2c_t<64>(g, s, out, n, tw, st, name);
+    case 128: return rows_r2c_t<128>(g, s, out, n, tw, st, name);
+    case 256: return rows_r2c_t<256>(g, s, out, n, tw, st, name);
+    case 512: return rows_r2c_t<512>(g, s, out, n, tw, st, name);
     default: return hipErrorInvalidValue;
   }
 }
 
-hipError_t launch_rows_r2c(const Geo& g, const RowSrc& s, float2* out, const float2* tw, hipStream_t st) {
-  return launch_rows_r2c_multi(g, &s, &out, 1, tw, st);
+hipError_t launch_rows_r2c_multi(const Geo& g, const RowSrc* s, float2* const* out, int n, const float2* tw,
+                                 hipStream_t st) {
+  return rows_r2c_any(g, s, out, n, tw, st, nullptr);
+}
+
+hipError_t launch_rows_r2c(const Geo& g, const RowSrc& s, float2* out, const float2* tw, hipStream_t st,
+                           const char* name) {
+  return rows_r2c_any(g, &s, &out, 1, tw, st, name);
 }
 
 // ----------------------------------------------------------------------------
@@ -405,6 +419,11 @@ __device__ __forceinline__ void cols_body(Geo g, ColIn a, ColIn b, ColIn c3, Col
   const int pm = pk / g.csf1;
   const int grp = pfr * g.csf1 + (pk - pm * g.csf1);
   const int row0 = pm * g.hop0;
+  const ColCtx cx{pm, pk - pm * g.csf1, pfr, g.hop0, g.Tw, tw};
+  // operands that may be shared image rows: the forward's and the spectra's
+  // first, the f-adjoint's second, the two-adjoint pass's third
+  constexpr bool XA = OP == COL_FWD || OP == COL_SPEC;
+  constexpr bool XB = OP == COL_ADJ1;
   float2* sl = sm + cc * SL;
 
   Twiddles<TH> T;
@@ -417,10 +436,10 @@ __device__ __forceinline__ void cols_body(Geo g, ColIn a, ColIn b, ColIn c3, Col
   // the 512-point two-adjoint pass, whose registers the extra paths spill.
   constexpr bool PRUNE = !(TH >= 512 && OP == COL_ADJ2);
   if (PRUNE && (OP == COL_FWD || OP == COL_SPEC) && !a.spec && a.rlo + a.nr <= 3 * TPT) {
-    col_load<TH, 3>(A, a, p, c, NB, t, active, grp, row0);
+    col_load<TH, 3, XA>(A, a, p, c, NB, t, active, grp, row0, cx);
     if (wact) fft<TH, -1, 3>(A, sl, t, T, tw);
   } else {
-    col_load<TH>(A, a, p, c, NB, t, active, grp, row0);
+    col_load<TH, 16, XA>(A, a, p, c, NB, t, active, grp, row0, cx);
     if (!a.spec && wact) fft<TH, -1>(A, sl, t, T, tw);
   }
   if constexpr (OP == COL_SPEC) {
@@ -437,10 +456,10 @@ __device__ __forceinline__ void cols_body(Geo g, ColIn a, ColIn b, ColIn c3, Col
   } else {
     float2 B[16];
     if (PRUNE && !b.spec && b.rlo + b.nr <= 3 * TPT) {
-      col_load<TH, 3>(B, b, p, c, NB, t, active, grp, row0);
+      col_load<TH, 3, XB>(B, b, p, c, NB, t, active, grp, row0, cx);
       if (wact) fft<TH, -1, 3>(B, sl, t, T, tw);
     } else {
-      col_load<TH>(B, b, p, c, NB, t, active, grp, row0);
+      col_load<TH, 16, XB>(B, b, p, c, NB, t, active, grp, row0, cx);
       if (!b.spec && wact) fft<TH, -1>(B, sl, t, T, tw);
     }
     {
@@ -455,7 +474,7 @@ __device__ __forceinline__ void cols_body(Geo g, ColIn a, ColIn b, ColIn c3, Col
       col_store_rows<TH, CS, OP != COL_ADJ2, 16, WGC>(B, sm, o1, p, strip, NB, g.NBP, cc, t);
     }
     if constexpr (OP == COL_ADJ2) {
-      col_load<TH>(B, c3, p, c, NB, t, active, grp, row0);
+      col_load<TH, 16, true>(B, c3, p, c, NB, t, active, grp, row0, cx);
       if (!c3.spec && wact) fft<TH, -1>(B, sl, t, T, tw);
       {
         constexpr bool SPREAD = SPREAD2;
@@ -1465,8 +1484,8 @@ __global__ __launch_bounds__(WG) void k_batch_rows(const BatchGroup* __restrict_
   const int gi = batch_find(L, (int)blockIdx.x);
   const int lb = (int)blockIdx.x - L.start[gi];
   const BatchGroup& B = G[L.gidx[gi]];
-  const Geo& g = res ? B.gres : B.g;
-  const RowJobs& jobs = res ? B.rows_res : B.rows_fwd;
+  const Geo& g = res == 1 ? B.gres : res == 2 ? B.gx : B.g;
+  const RowJobs& jobs = res == 1 ? B.rows_res : res == 2 ? B.rows_x : B.rows_fwd;
   if (lb >= g.P * jobs.blk_end[jobs.n - 1]) return;
   int pu, ji, q0;
   rows_unit<TW>(g, jobs, lb, pu, ji, q0);
@@ -1546,6 +1565,7 @@ int batch_blocks(const BatchGroup& G, int pass) {
   switch (pass) {
     case BP_ROWS_FWD: return G.g.P * G.rows_fwd.blk_end[G.rows_fwd.n - 1];
     case BP_ROWS_RES: return G.gres.P * G.rows_res.blk_end[G.rows_res.n - 1];
+    case BP_ROWS_X: return G.rows_x.n ? G.gx.P * G.rows_x.blk_end[G.rows_x.n - 1] : 0;
     case BP_COLS_FWD:
     case BP_COLS_ADJ: {
       const int CS = WG / (G.g.Th / 16);
@@ -1594,12 +1614,13 @@ static hipError_t batch_t(const BatchGroup* G, const BatchLaunch& L, int pass, i
                           hipStream_t st) {
   const dim3 grid((unsigned)L.start[L.ng]);
   if (grid.x == 0) return hipSuccess;
-  static const char* names[BP_N] = {"batch_rows_fwd", "batch_cols_fwd", "batch_ola_fwd", "batch_rows_res",
+  static const char* names[BP_N] = {"batch_rows_fwd", "batch_rows_x", "batch_cols_fwd", "batch_ola_fwd", "batch_rows_res",
                                     "batch_cols_adj2", "batch_ola_win", "batch_stack"};
   ProfScope ps(names[pass], st);
   switch (pass) {
     case BP_ROWS_FWD: hipLaunchKernelGGL(k_batch_rows<T>, grid, dim3(WG), 0, st, G, L, 0); break;
     case BP_ROWS_RES: hipLaunchKernelGGL(k_batch_rows<T>, grid, dim3(WG), 0, st, G, L, 1); break;
+    case BP_ROWS_X: hipLaunchKernelGGL(k_batch_rows<T>, grid, dim3(WG), 0, st, G, L, 2); break;
     case BP_COLS_FWD: hipLaunchKernelGGL((k_batch_cols<T, COL_FWD>), grid, dim3(WG), 0, st, G, L); break;
     case BP_COLS_ADJ: hipLaunchKernelGGL((k_batch_cols<T, COL_ADJ2>), grid, dim3(WG), 0, st, G, L); break;
     case BP_OLA_FWD: batch_ola_launch<T, false>(G, L, wgs, shm, st, grid); break;
