@@ -438,6 +438,10 @@ __device__ __forceinline__ void cols_body(Geo g, ColIn a, ColIn b, ColIn c3, Col
   if (PRUNE && (OP == COL_FWD || OP == COL_SPEC) && !a.spec && a.rlo + a.nr <= 3 * TPT) {
     col_load<TH, 3, XA>(A, a, p, c, NB, t, active, grp, row0, cx);
     if (wact) fft<TH, -1, 3>(A, sl, t, T, tw);
+  } else if (PRUNE && XA && !a.spec && a.rlo + a.nr <= 12 * TPT) {
+    // image rows of a patch of at most 3/4 of the tile height
+    col_load<TH, 12, XA>(A, a, p, c, NB, t, active, grp, row0, cx);
+    if (wact) fft<TH, -1, 12>(A, sl, t, T, tw);
   } else {
     col_load<TH, 16, XA>(A, a, p, c, NB, t, active, grp, row0, cx);
     if (!a.spec && wact) fft<TH, -1>(A, sl, t, T, tw);
@@ -474,8 +478,13 @@ __device__ __forceinline__ void cols_body(Geo g, ColIn a, ColIn b, ColIn c3, Col
       col_store_rows<TH, CS, OP != COL_ADJ2, 16, WGC>(B, sm, o1, p, strip, NB, g.NBP, cc, t);
     }
     if constexpr (OP == COL_ADJ2) {
-      col_load<TH, 16, true>(B, c3, p, c, NB, t, active, grp, row0, cx);
-      if (!c3.spec && wact) fft<TH, -1>(B, sl, t, T, tw);
+      if (PRUNE && !c3.spec && c3.rlo + c3.nr <= 12 * TPT) {
+        col_load<TH, 12, true>(B, c3, p, c, NB, t, active, grp, row0, cx);
+        if (wact) fft<TH, -1, 12>(B, sl, t, T, tw);
+      } else {
+        col_load<TH, 16, true>(B, c3, p, c, NB, t, active, grp, row0, cx);
+        if (!c3.spec && wact) fft<TH, -1>(B, sl, t, T, tw);
+      }
       {
         constexpr bool SPREAD = SPREAD2;
         VMBD_COL_PRODUCT(true, A, B);
