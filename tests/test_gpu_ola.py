@@ -383,3 +383,43 @@ def test_separable_windows(geom, cuda):
     assert not p_bad.separable
     assert torch.equal(p_bad.adjoint_x(gt, ft), p_bad2d.adjoint_x(gt, ft))
 
+
+@pytest.mark.parametrize('geom', GEOMS + [((90, 60), (3, 1), (7, 7), 0.5), ((60, 90), (1, 3), (7, 7), 0.5)],
+                         ids=lambda g: f'{g[0]}-{g[1]}-{g[2]}')
+def test_shared_image_rows(geom, cuda):
+    """Separable windows: the image rows are transformed once per patch
+    column and scaled per patch row in the column pass, with the floor
+    columns' own terms added there.  Forward, f-adjoint and both adjoints
+    against the oracle and, pixel by pixel, against the per-patch image rows
+    of the two-dimensional path (single patch rows and columns included,
+    where one patch column holds both floor columns)."""
+    import torch
+    from vmbd_amd.imagetools import win2winaux
+    from vmbd_amd.ola import OlaPlan
+    sx, csf, sf, ov = geom
+    print(f'case shared image rows sx={sx} csf={csf} sf={sf} ov={ov}', flush=True)
+    wa = win2winaux(sx, csf, ov)
+    owa = O.Win2WinAux(sx, csf, ov)
+    p_sep = OlaPlan(sx, sf, wa, device=cuda)
+    p_2d = OlaPlan(sx, sf, wa, device=cuda, separable=False)
+    assert p_sep.separable and not p_2d.separable
+    x, f, g = case_inputs(11, sx, csf, sf)
+    x32, f32, g32 = (np.float32(a).astype(np.float64) for a in (x, f, g))
+    xt, ft, gt = _t(x, cuda), _t(f, cuda), _t(g, cuda)
+
+    def close(a, b, ref):
+        a, b = a.cpu().numpy(), b.cpu().numpy()
+        assert rel_l2(a, ref) <= TOL
+        assert np.abs(a - b).max() <= 1e-5 * np.abs(ref).max()
+
+    close(p_sep.forward(xt, ft), p_2d.forward(xt, ft), O.forward(x32, f32, owa))
+    fref = O.adjoint_f(g32, x32, owa, sf, exact=True)
+    close(p_sep.adjoint_f(gt, xt, exact=True), p_2d.adjoint_f(gt, xt, exact=True), fref)
+    # the pair reusing the forward's shared rows
+    p_sep.forward(xt, ft)
+    p_2d.forward(xt, ft)
+    xa, fa = p_sep.adjoint_xf(gt, xt, ft, exact=True)
+    xb, fb = p_2d.adjoint_xf(gt, xt, ft, exact=True)
+    close(xa, xb, O.adjoint_x(g32, f32, owa, sx, exact=True))
+    close(fa, fb, fref)
+    torch.cuda.synchronize()

@@ -29,7 +29,7 @@ struct vmbd_plan {
   int nslot = 0;
   int n_win = 0;
   float2* wsG = nullptr;  // [frames][csf1][NB][row_pitch((csf0-1)*hop0 + ext0)] shared residual rows
-  float2* wsX = nullptr;  // [P][NB][row_pitch(sw0)]
+  float2* wsX = nullptr;  // [P][NB][row_pitch(sw0)], or shared image rows [frames][csf1][NB][row_pitch(S0)]
   float2* wsF = nullptr;  // [P][NB][row_pitch(sf0)]
   float2* Y1 = nullptr;   // [P][ext0][NBP]
   float2* Y2 = nullptr;   // [P][sf0][NBP]
@@ -543,6 +543,10 @@ int vmbd_plan_set_window_factors(vmbd_plan* p, const float* a, const float* b, f
   const Geo& g = p->g;
   const int sw0 = g.sw0, sw1 = g.sw1, hop0 = g.hop0, hop1 = g.hop1;
   const int S0 = (g.csf0 - 1) * hop0 + sw0, S1 = (g.csf1 - 1) * hop1 + sw1;  // overlap-add canvas
+  // the shared image rows ([frames][csf1][NB][row_pitch(S0)]) live in the
+  // per-patch image-row workspace; it holds them whenever hop0 <= sw0
+  if ((size_t)g.csf1 * row_pitch(S0) > (size_t)g.Pf * row_pitch(sw0))
+    return vmbd_fail(VMBD_E_UNSUP, "vmbd_plan_set_window_factors: shared image rows exceed the row workspace");
   // nom and its overlap-add in FP64
   std::vector<double> nom((size_t)sw0 * sw1), wimg((size_t)S0 * S1, 0.0);
   for (int r = 0; r < sw0; ++r)
