@@ -83,10 +83,16 @@ typedef struct vmbd_plan vmbd_plan;
  *                every row's transforms in one round of that many threads)
  *   col_threads  threads per workgroup of the column passes: 128 or 256
  *                (a workgroup carries 16 * threads / Th columns of one
- *                patch, i.e. threads / 64 wavefronts per column strip) */
+ *                patch, i.e. threads / 64 wavefronts per column strip)
+ *   image_rows   with separable windows (vmbd_plan_set_window_factors):
+ *                1 transforms each image row once per patch column and
+ *                applies the patch rows' factors in the column pass, 2
+ *                once per patch, 0 chooses by launch size.  The two forms
+ *                round differently (each within the operators' accuracy). */
 typedef struct vmbd_tuning {
   int32_t ola_threads;
   int32_t col_threads;
+  int32_t image_rows;
 } vmbd_tuning;
 
 /* Library version (major*10000 + minor*100 + patch). */
@@ -110,7 +116,9 @@ vmbd_plan* vmbd_plan_create(const vmbd_geom* geom, const float* win_table, int32
 void vmbd_plan_destroy(vmbd_plan* plan);
 
 /* Set / read the plan's launch shapes (VMBD_E_UNSUP for a shape the
- * plan's geometry cannot use). */
+ * plan's geometry cannot use, or image_rows = 1 without separable
+ * windows).  Setting them drops the row transforms a forward left for
+ * VMBD_REUSE_ROWS. */
 int vmbd_plan_set_tuning(vmbd_plan* plan, const vmbd_tuning* tuning);
 int vmbd_plan_get_tuning(const vmbd_plan* plan, vmbd_tuning* tuning);
 
@@ -122,10 +130,19 @@ int vmbd_plan_get_tuning(const vmbd_plan* plan, vmbd_tuning* tuning);
  * overlap-add in summed form: sum_n b(c) IFFT(sum_m a(r) Y[m, n][r]) / wimg,
  * with the pixels whose covering windows are all at the floor (the outer
  * rows and columns of the patch grid) taken as the reference takes them,
- * the mean of the covering patches.  VMBD_E_UNSUP when the tables do not
- * factor (the plan keeps its two-dimensional windows).  Allocates; call
- * once after vmbd_plan_create.  No counterpart in the reference's API. */
+ * the mean of the covering patches.  Launches large enough to fill the GPU
+ * several times over also transform each image row once per patch column
+ * (instead of once per covering patch) and apply the patch rows' factors
+ * in the column pass.  VMBD_E_UNSUP when the tables do not factor (the
+ * plan keeps its two-dimensional windows).  Allocates; call once after
+ * vmbd_plan_create.  No counterpart in the reference's API. */
 int vmbd_plan_set_window_factors(vmbd_plan* plan, const float* a, const float* b, float floor);
+
+/* 1 when calls of nframes frames transform the image rows once per patch
+ * column (separable windows, large launches), 0 when once per patch, < 0
+ * on a bad argument.  The forward then runs the image and the PSF rows as
+ * two launches instead of one.  No counterpart in the reference's API. */
+int vmbd_plan_image_rows_shared(const vmbd_plan* plan, int32_t nframes);
 
 /* Device bytes held by the plan (workspaces, windows, twiddles). */
 size_t vmbd_plan_workspace_bytes(const vmbd_plan* plan);

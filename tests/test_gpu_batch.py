@@ -25,30 +25,41 @@ def _frames(cuda):
     return xs, fs, gs
 
 
+@pytest.mark.parametrize('image_rows', [0, 1], ids=['auto-rows', 'shared-rows'])
 @pytest.mark.parametrize('grouped', [True, False], ids=['grouped', 'per-group'])
-def test_mixed_batch_vs_oracle_and_single_plans(cuda, grouped):
+def test_mixed_batch_vs_oracle_and_single_plans(cuda, grouped, image_rows):
     import torch
     from vmbd_amd.batch import MixedBatch
     from vmbd_amd.imagetools import win2winaux
     from vmbd_amd.ola import OlaPlan
-    print(f'case mixed batch grouped={grouped}', flush=True)
+    print(f'case mixed batch grouped={grouped} image_rows={image_rows}', flush=True)
     xs, fs, gs = _frames(cuda)
     mb = MixedBatch(SPECS, grouped=grouped)
     assert len(mb.groups) == 3
-    # separable windows give the image rows a pass of their own (eight
+    for G in mb.groups:  # before the library batches are bound
+        G['plan'].set_tuning(image_rows=image_rows)
+    # image rows shared per patch column get a row pass of their own (eight
     # launches per step instead of seven)
-    sep = int(mb.groups[0]['plan'].separable)
-    assert all(G['plan'].separable == bool(sep) for G in mb.groups)
-    # grouped over three streams: one batch per tile, 7 + sep launches each
-    assert mb.launches == ((7 + sep) * len({G['plan'].tile for G in mb.groups}) if grouped else 3 * (7 + sep))
+    shared = [G['plan'].image_rows_shared(len(G['idx'])) for G in mb.groups]
+    tiles = {G['plan'].tile for G in mb.groups}
+    if grouped:
+        # grouped over three streams: one batch per tile, 7 launches each
+        # plus one where a group of the tile shares its image rows
+        assert mb.launches == sum(7 + any(sh for G, sh in zip(mb.groups, shared) if G['plan'].tile == t)
+                                  for t in tiles)
+    else:
+        assert mb.launches == sum(7 + sh for sh in shared)
     if grouped:
         # on one stream one batch of all groups: one launch per pass and
-        # distinct tile (width for the 5 + sep row, overlap-add and stack
-        # passes, height for the two column passes)
+        # distinct tile (width for the five row, overlap-add and stack
+        # passes and the shared image rows, height for the two column passes)
         m1 = MixedBatch(SPECS, grouped=True, streams=1)
+        for G in m1.groups:
+            G['plan'].set_tuning(image_rows=image_rows)
         n_tw = len({G['plan'].tile[1] for G in m1.groups})
         n_th = len({G['plan'].tile[0] for G in m1.groups})
-        assert m1.launches == (5 + sep) * n_tw + 2 * n_th
+        n_x = len({G['plan'].tile[1] for G in m1.groups if G['plan'].image_rows_shared(len(G['idx']))})
+        assert m1.launches == 5 * n_tw + n_x + 2 * n_th
     mb.load(xs, fs, gs)
     mb.step()
     torch.cuda.synchronize()
@@ -61,6 +72,7 @@ def test_mixed_batch_vs_oracle_and_single_plans(cuda, grouped):
         assert rel_l2(fb.cpu().numpy(), O.adjoint_f(g64, x64, owa, sf, exact=True)) < 1e-5, i
         # batching changes nothing: the single-frame plan gives the same bits
         p1 = OlaPlan(sx, sf, win2winaux(sx, csf, 0.5), 'valid')
+        p1.set_tuning(image_rows=image_rows)
         assert torch.equal(p1.forward(xs[i], fs[i]), y), i
         xb1, fb1 = p1.adjoint_xf(gs[i], xs[i], fs[i], exact=True)
         assert torch.equal(xb1, xb) and torch.equal(fb1, fb), i
@@ -138,7 +150,7 @@ def test_launch_shapes_same_bits_and_tuner(cuda, tmp_path, monkeypatch):
             except _lib.VmbdError as e:
                 assert e.code == _lib.VMBD_E_UNSUP and shape[0] in (128, 192), (shape, e)
                 continue
-            assert p.tuning == dict(ola_threads=shape[0], col_threads=shape[1])
+            assert p.tuning == dict(ola_threads=shape[0], col_threads=shape[1], image_rows=0)
             out = (p.forward(t(x), t(f)),) + p.adjoint_xf(t(g), t(x), t(f))
             if ref is None:
                 ref = out

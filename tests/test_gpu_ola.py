@@ -394,6 +394,7 @@ def test_shared_image_rows(geom, cuda):
     of the two-dimensional path (single patch rows and columns included,
     where one patch column holds both floor columns)."""
     import torch
+    from vmbd_amd import _lib
     from vmbd_amd.imagetools import win2winaux
     from vmbd_amd.ola import OlaPlan
     sx, csf, sf, ov = geom
@@ -403,6 +404,13 @@ def test_shared_image_rows(geom, cuda):
     p_sep = OlaPlan(sx, sf, wa, device=cuda)
     p_2d = OlaPlan(sx, sf, wa, device=cuda, separable=False)
     assert p_sep.separable and not p_2d.separable
+    # small launches keep one transform per patch unless told otherwise
+    assert not p_sep.image_rows_shared(1)
+    p_sep.set_tuning(image_rows=1)
+    assert p_sep.image_rows_shared(1) and p_sep.tuning['image_rows'] == 1
+    with pytest.raises(_lib.VmbdError) as e:
+        p_2d.set_tuning(image_rows=1)
+    assert e.value.code == _lib.VMBD_E_UNSUP
     x, f, g = case_inputs(11, sx, csf, sf)
     x32, f32, g32 = (np.float32(a).astype(np.float64) for a in (x, f, g))
     xt, ft, gt = _t(x, cuda), _t(f, cuda), _t(g, cuda)
@@ -412,14 +420,19 @@ def test_shared_image_rows(geom, cuda):
         assert rel_l2(a, ref) <= TOL
         assert np.abs(a - b).max() <= 1e-5 * np.abs(ref).max()
 
-    close(p_sep.forward(xt, ft), p_2d.forward(xt, ft), O.forward(x32, f32, owa))
+    def run(p):
+        y = p.forward(xt, ft)
+        fo = p.adjoint_f(gt, xt, exact=True)
+        # the pair reusing the forward's rows
+        p.forward(xt, ft)
+        xa, fa = p.adjoint_xf(gt, xt, ft, exact=True)
+        return y, fo, xa, fa
+
+    a, b = run(p_sep), run(p_2d)
     fref = O.adjoint_f(g32, x32, owa, sf, exact=True)
-    close(p_sep.adjoint_f(gt, xt, exact=True), p_2d.adjoint_f(gt, xt, exact=True), fref)
-    # the pair reusing the forward's shared rows
-    p_sep.forward(xt, ft)
-    p_2d.forward(xt, ft)
-    xa, fa = p_sep.adjoint_xf(gt, xt, ft, exact=True)
-    xb, fb = p_2d.adjoint_xf(gt, xt, ft, exact=True)
-    close(xa, xb, O.adjoint_x(g32, f32, owa, sx, exact=True))
-    close(fa, fb, fref)
+    for u, v, ref in zip(a, b, (O.forward(x32, f32, owa), fref, O.adjoint_x(g32, f32, owa, sx, exact=True), fref)):
+        close(u, v, ref)
+    # two wavefronts per column strip: the same bits
+    p_sep.set_tuning(col_threads=128, image_rows=1)
+    assert all(torch.equal(u, v) for u, v in zip(run(p_sep), a))
     torch.cuda.synchronize()

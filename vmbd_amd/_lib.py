@@ -22,7 +22,7 @@ REUSE_ROWS = 1
 
 
 class VmbdTuning(ctypes.Structure):
-    _fields_ = [('ola_threads', ctypes.c_int32), ('col_threads', ctypes.c_int32)]
+    _fields_ = [('ola_threads', ctypes.c_int32), ('col_threads', ctypes.c_int32), ('image_rows', ctypes.c_int32)]
 
 
 class VmbdGeom(ctypes.Structure):
@@ -47,6 +47,7 @@ SIGNATURES = {
     'vmbd_plan_set_tuning': (ctypes.c_int, [_P, ctypes.POINTER(VmbdTuning)]),
     'vmbd_plan_get_tuning': (ctypes.c_int, [_P, ctypes.POINTER(VmbdTuning)]),
     'vmbd_plan_set_window_factors': (ctypes.c_int, [_P, _P, _P, ctypes.c_float]),
+    'vmbd_plan_image_rows_shared': (ctypes.c_int, [_P, _I]),
     'vmbd_plan_workspace_bytes': (ctypes.c_size_t, [_P]),
     'vmbd_plan_out_shape': (ctypes.c_int, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
     'vmbd_plan_spectrum_elems': (ctypes.c_size_t, [_P]),

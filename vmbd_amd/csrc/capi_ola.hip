@@ -222,16 +222,33 @@ ColIn shared_in(const float2* ws, const SharedRows& r) {
 // columns' own terms): the rows of one image row are the same for every
 // patch row that covers it, up to that scale.
 struct XRows {
-  Geo g;     // row-pass geometry
+  Geo g;        // row-pass geometry
   RowSrc s;
-  ColIn in;  // the column pass's operand
+  ColIn in;     // the column pass's operand
+  bool shared;  // one transform per canvas row and patch column
 };
+
+// Per-patch image-row workgroups at or above which the shared rows are
+// used.  Below it the launches are too small to fill the GPU several times
+// over, and the column pass's factors lengthen the kernels' latency more
+// than the fewer row transforms save (1024^2 frames, 10x10 patches, tile
+// 256: 600 workgroups per frame; 1 and 2 frames run faster per patch, 4
+// and more shared).
+constexpr long long XS_MIN_BLOCKS = 2048;
+
+bool shared_x(const vmbd_plan* p, const Geo& g) {
+  if (!p->xs || p->tune.image_rows == 2) return false;
+  if (p->tune.image_rows == 1) return true;
+  const int TPW = 256 / (g.Tw / 16);
+  return (long long)g.P * (((g.sw0 + 1) / 2 + TPW - 1) / TPW) >= XS_MIN_BLOCKS;
+}
 
 XRows x_rows(const vmbd_plan* p, const Geo& g, const float* x) {
   XRows r;
   r.g = g;
   r.s = src_image(p, x);
-  if (!p->xs) {
+  r.shared = shared_x(p, g);
+  if (!r.shared) {
     r.in = raw_in(p->wsX, r.s);
     return r;
   }
@@ -320,7 +337,7 @@ OlaDst dst_adjoint_x(const vmbd_plan* p, float* xbar) {
 // row transforms of x and f: one launch over both (one geometry), or, with
 // the image rows shared per patch column, one launch each
 hipError_t launch_xf_rows(const vmbd_plan* p, const Geo& g, const XRows& xr, const RowSrc& sf, hipStream_t st) {
-  if (!p->xs) {
+  if (!xr.shared) {
     RowSrc srcs[2] = {xr.s, sf};
     float2* outs[2] = {p->wsX, p->wsF};
     return launch_rows_r2c_multi(g, srcs, outs, 2, p->tw, st);
@@ -531,7 +548,12 @@ int vmbd_plan_set_tuning(vmbd_plan* p, const vmbd_tuning* t) {
   if ((o == 128 || o == 192) && !(ola_fits(p->g, dst_forward(p, nullptr), o) &&
                                   ola_fits(p->g, dst_adjoint_x(p, nullptr), o)))
     return vmbd_fail(VMBD_E_UNSUP, "vmbd_plan_set_tuning: an output row needs more than one round of %d threads", o);
+  if (t->image_rows < 0 || t->image_rows > 2)
+    return vmbd_fail(VMBD_E_ARG, "vmbd_plan_set_tuning: image_rows=%d not in {0, 1, 2}", t->image_rows);
+  if (t->image_rows == 1 && !p->xs)
+    return vmbd_fail(VMBD_E_UNSUP, "vmbd_plan_set_tuning: image_rows=1 needs separable windows");
   p->tune = *t;
+  p->rows_frames = 0;  // the image-row layout may change
   return VMBD_OK;
 }
 
@@ -708,6 +730,14 @@ int vmbd_plan_set_window_factors(vmbd_plan* p, const float* a, const float* b, f
   p->xs = 1;          // image rows shared per patch column from now on
   p->rows_frames = 0;  // the workspaces hold no rows of the new layout
   return VMBD_OK;
+}
+
+int vmbd_plan_image_rows_shared(const vmbd_plan* p, int32_t nframes) {
+  vmbd_clear_error();
+  if (!p) return vmbd_fail(VMBD_E_ARG, "vmbd_plan_image_rows_shared: null plan");
+  if (nframes < 1 || nframes > p->max_frames)
+    return vmbd_fail(VMBD_E_ARG, "nframes=%d outside [1, max_frames=%d]", nframes, p->max_frames);
+  return shared_x(p, call_geo(p, nframes)) ? 1 : 0;
 }
 
 int vmbd_plan_get_tuning(const vmbd_plan* p, vmbd_tuning* t) {
@@ -1161,7 +1191,7 @@ BatchGroup batch_group(vmbd_plan* p, int nframes, const float* x, const float* f
   const RowSrc sf = src_psf(p, f);
   const int TPW = WG / (g.Tw / 16);
   auto blocks = [&](const RowSrc& s) { return ((s.rhi - s.rlo + 1) / 2 + TPW - 1) / TPW; };
-  if (p->xs) {  // the image rows in a pass of their own (their geometry)
+  if (xr.shared) {  // the image rows in a pass of their own (their geometry)
     B.rows_fwd.n = 1;
     B.rows_fwd.s[0] = sf;
     B.rows_fwd.out[0] = p->wsF;

@@ -503,36 +503,15 @@ __device__ __forceinline__ void cols_body(Geo g, ColIn a, ColIn b, ColIn c3, Col
 // workgroups per CU.  Strips of one patch are consecutive workgroups; each
 // writes whole 128-byte lines of its rows (16 bins, row pitch NBP a multiple
 // of 16).
-// 1-D grid over work units (patch, strip), P * strips of them.  Workgroup
-// ids are dealt round-robin to the NXCD XCDs; XCD j runs the contiguous
-// units [start_j, start_j + count_j), ordered frame, patch column, patch row,
-// strip (strip fastest): the patches of one patch column follow each other
-// on one XCD, so the rows they share (shared image and residual rows, each
-// row read by about ext0/hop0 patches) are read from its L2.  The map
-// blk -> (patch, strip) is a bijection whatever the hardware placement.
-__device__ __forceinline__ void cols_unit(const Geo& g, int strips, int blk, int& strip, int& p) {
-  const int total = g.P * strips;
-  const int xcd = blk % NXCD;
-  const int u = xcd * (total / NXCD) + min(xcd, total % NXCD) + blk / NXCD;
-  const int r = u / strips;
-  strip = u - r * strips;
-  const int fn = r / g.csf0;  // frame * csf1 + n
-  const int m = r - fn * g.csf0;
-  const int fr = fn / g.csf1;
-  const int n = fn - fr * g.csf1;
-  p = fr * g.Pf + m * g.csf1 + n;
-}
-
 template <int TH, int OP, int WGC>
 __global__ __launch_bounds__(WGC, (TH >= 256 ? 4 * WG / WGC : 1)) void k_cols(Geo g, ColIn a, ColIn b, ColIn c3,
                                                                              ColOut o1, ColOut o2,
                                                                              float2* __restrict__ spec_out,
                                                                              const float2* __restrict__ tw) {
   __shared__ float2 sm[ColsLds<TH, OP, WGC>::v];
-  constexpr int CS = WGC / (TH / 16);
-  int strip, p;
-  cols_unit(g, (g.NB - 1 + CS - 1) / CS, (int)blockIdx.x, strip, p);
-  cols_body<TH, OP, WGC>(g, a, b, c3, o1, o2, spec_out, strip, p, sm, tw);
+  // frame blockIdx.z, patch blockIdx.y
+  cols_body<TH, OP, WGC>(g, a, b, c3, o1, o2, spec_out, blockIdx.x, (int)blockIdx.z * g.Pf + (int)blockIdx.y, sm,
+                         tw);
 }
 
 #undef VMBD_COL_PRODUCT
@@ -542,7 +521,7 @@ static hipError_t cols_wg(const Geo& g, int op, const ColIn& a, const ColIn& b, 
                           const ColOut& o2, float2* spec_out, const float2* tw, hipStream_t st) {
   constexpr int CS = WGC / (TH / 16);
   const int strips = (g.NB - 1 + CS - 1) / CS;  // bins [0, NB-1): NB-1 is packed into bin 0
-  const dim3 grid((unsigned)(strips * g.P));
+  const dim3 grid((unsigned)strips, (unsigned)g.Pf, (unsigned)(g.P / g.Pf));
   static const char* names[4] = {"cols_spec", "cols_fwd", "cols_adj1", "cols_adj2"};
   ProfScope ps(names[op & 3], st);
   switch (op) {
@@ -1532,13 +1511,12 @@ __global__ __launch_bounds__(WG, (TH >= 256 ? 4 : 1)) void k_batch_cols(const Ba
   const int lb = (int)blockIdx.x - L.start[gi];
   const BatchGroup& B = G[L.gidx[gi]];
   const int strips = (B.g.NB - 1 + CS - 1) / CS;
-  if (lb >= B.g.P * strips) return;
-  int strip, p;
-  cols_unit(B.g, strips, lb, strip, p);
+  const int p = lb / strips;
+  if (p >= B.g.P) return;
   if constexpr (OP == COL_FWD)
-    cols_body<TH, COL_FWD>(B.g, B.cf_a, B.cf_b, B.cf_b, B.cf_o, B.cf_o, nullptr, strip, p, sm, B.tw);
+    cols_body<TH, COL_FWD>(B.g, B.cf_a, B.cf_b, B.cf_b, B.cf_o, B.cf_o, nullptr, lb - p * strips, p, sm, B.tw);
   else
-    cols_body<TH, COL_ADJ2>(B.g, B.ca_a, B.ca_b, B.ca_c, B.ca_o1, B.ca_o2, nullptr, strip, p, sm, B.tw);
+    cols_body<TH, COL_ADJ2>(B.g, B.ca_a, B.ca_b, B.ca_c, B.ca_o1, B.ca_o2, nullptr, lb - p * strips, p, sm, B.tw);
 }
 
 // WGS < WG: every group of the launch holds its row in one round of WGS

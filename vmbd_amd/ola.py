@@ -89,19 +89,33 @@ class OlaPlan:
             elif rc != _lib.VMBD_E_UNSUP:
                 _lib.check(rc, 'vmbd_plan_set_window_factors')
 
+    def image_rows_shared(self, nframes=1):
+        """True when calls of `nframes` frames transform each image row once
+        per patch column (separable windows and launches large enough to
+        fill the GPU several times over); the forward then runs its image
+        and PSF rows as two launches."""
+        rc = _lib.lib().vmbd_plan_image_rows_shared(self._h, int(nframes))
+        if rc < 0:
+            _lib.check(rc, 'vmbd_plan_image_rows_shared')
+        return rc == 1
+
     # -- launch shapes --------------------------------------------------------
-    def set_tuning(self, ola_threads=0, col_threads=0):
+    def set_tuning(self, ola_threads=0, col_threads=0, image_rows=0):
         """Threads per workgroup of the overlap-add passes (128 / 192 / 256)
         and of the column passes (128 / 256); 0 = automatic.  Results are
-        the same bits for every setting."""
-        t = _lib.VmbdTuning(int(ola_threads), int(col_threads))
+        the same bits for every setting of these two.  image_rows (separable
+        windows only): 1 transforms each image row once per patch column,
+        2 once per patch, 0 chooses by launch size; the two forms round
+        differently, each within the operators' accuracy."""
+        t = _lib.VmbdTuning(int(ola_threads), int(col_threads), int(image_rows))
         _lib.check(_lib.lib().vmbd_plan_set_tuning(self._h, ctypes.byref(t)), 'set_tuning')
+        self._rows_key = None  # the library dropped the forward's row transforms
 
     @property
     def tuning(self):
         t = _lib.VmbdTuning()
         _lib.check(_lib.lib().vmbd_plan_get_tuning(self._h, ctypes.byref(t)), 'get_tuning')
-        return dict(ola_threads=t.ola_threads, col_threads=t.col_threads)
+        return dict(ola_threads=t.ola_threads, col_threads=t.col_threads, image_rows=t.image_rows)
 
     def __del__(self):
         h = getattr(self, '_h', None)
