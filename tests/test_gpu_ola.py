@@ -423,14 +423,22 @@ def test_shared_image_rows(geom, cuda):
     def run(p):
         y = p.forward(xt, ft)
         fo = p.adjoint_f(gt, xt, exact=True)
+        fl = p.adjoint_f(gt, xt, exact=False)
         # the pair reusing the forward's rows
         p.forward(xt, ft)
         xa, fa = p.adjoint_xf(gt, xt, ft, exact=True)
-        return y, fo, xa, fa
+        # cached spectra: the image's (its column spectra from the shared
+        # rows) and the PSFs' with the image as the row-pass operand
+        Xs, Fs = p.spectra('X', xt), p.spectra('F', ft)
+        return (y, fo, fl, xa, fa, p.forward_cached('X', Xs, ft), p.adjoint_cached('X', Xs, gt, exact=True),
+                p.adjoint_cached('X', Xs, gt, exact=False), p.forward_cached('F', Fs, xt))
 
     a, b = run(p_sep), run(p_2d)
+    yref = O.forward(x32, f32, owa)
     fref = O.adjoint_f(g32, x32, owa, sf, exact=True)
-    for u, v, ref in zip(a, b, (O.forward(x32, f32, owa), fref, O.adjoint_x(g32, f32, owa, sx, exact=True), fref)):
+    flref = O.adjoint_f(g32, x32, owa, sf, exact=False)
+    refs = (yref, fref, flref, O.adjoint_x(g32, f32, owa, sx, exact=True), fref, yref, fref, flref, yref)
+    for u, v, ref in zip(a, b, refs):
         close(u, v, ref)
     # two wavefronts per column strip: the same bits
     p_sep.set_tuning(col_threads=128, image_rows=1)
