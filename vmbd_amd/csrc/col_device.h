@@ -37,19 +37,16 @@ __device__ __forceinline__ void col_load(float2 (&v)[16], const ColIn& in, int p
 #pragma unroll
     for (int r = 0; r < 16; ++r) v[r] = active ? col[t + TPT * r] : make_float2(0.f, 0.f);
   } else if (XS && in.rsc) {
-    // shared image rows, each scaled by its patch row's factor.  Row q of
-    // the operand sits in lane q % 64 of table register q / 64 (one coalesced
-    // load per 64 rows) and reaches the lanes that hold it in a transform by
-    // a cross-lane permute: the loads of a column issue together.
-    constexpr int K = (TPT * NZ + 63) / 64;
-    const int lane = threadIdx.x & 63;
+    // shared image rows, each scaled by its patch row's factor.  The factors
+    // are loaded unconditionally (index clamped into the table, so no load
+    // waits behind a branch) and applied once the column's loads are in.
     const float2* col = in.p + ((size_t)grp * NB + c) * in.pitch + row0;
     const float* rs = in.rsc + (size_t)cx.pm * in.nr;
-    float tab[K];
+    float sc[NZ];
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const int q = 64 * k + lane - in.rlo;
-      tab[k] = (q >= 0 && q < in.nr) ? rs[q] : 0.f;
+    for (int r = 0; r < NZ; ++r) {
+      const int rr = t + TPT * r - in.rlo;
+      sc[r] = rs[min(max(rr, 0), in.nr - 1)];
     }
 #pragma unroll
     for (int r = 0; r < NZ; ++r) {
@@ -58,9 +55,8 @@ __device__ __forceinline__ void col_load(float2 (&v)[16], const ColIn& in, int p
     }
 #pragma unroll
     for (int r = 0; r < NZ; ++r) {
-      const float sc = __shfl(tab[(TPT * r) / 64], (TPT * r) % 64 + t, 64);
-      v[r].x *= sc;
-      v[r].y *= sc;
+      v[r].x *= sc[r];
+      v[r].y *= sc[r];
     }
     // floor columns at the canvas edge: the pixel's own term, at its row's
     // weight (no patch-row factor), as bin c of a unit impulse at column ec:
@@ -72,6 +68,12 @@ __device__ __forceinline__ void col_load(float2 (&v)[16], const ColIn& in, int p
       const int en = e ? in.en[1] : in.en[0];
       const int ec = e ? in.ec[1] : in.ec[0];
       if (in.ex && ecol >= 0 && cx.pn == en) {
+        // row q of the operand's term sits in lane q % 64 of table register
+        // q / 64 (one gather per 64 rows) and reaches the lanes that hold it
+        // in a transform by a cross-lane permute
+        constexpr int K = (TPT * NZ + 63) / 64;
+        const int lane = threadIdx.x & 63;
+        float tab[K];
         const float* xf = in.ex + (size_t)cx.pfr * in.exf + ecol;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
