@@ -27,11 +27,15 @@ def _t(a, cuda):
 
 @pytest.mark.parametrize('geom', GEOMS, ids=lambda g: f'{g[0]}-{g[1]}-{g[2]}')
 @pytest.mark.parametrize('mode', ['valid', 'same', 'full'])
-def test_forward_and_adjoints(geom, mode, cuda):
+@pytest.mark.parametrize('image_rows', [0, 1], ids=['rows-by-size', 'rows-shared'])
+def test_forward_and_adjoints(geom, mode, image_rows, cuda):
     import torch
     sx, csf, sf, ov = geom
-    print(f'case sx={sx} csf={csf} sf={sf} ov={ov} mode={mode}', flush=True)
+    print(f'case sx={sx} csf={csf} sf={sf} ov={ov} mode={mode} image_rows={image_rows}', flush=True)
     wa, plan, owa = _mk(sx, csf, ov, sf, mode, cuda)
+    # 1: the image rows transformed once per patch column (these launches
+    # are small enough to take one transform per patch by default)
+    plan.set_tuning(image_rows=image_rows)
     x, f, g = case_inputs(1, sx, csf, sf, mode)
     # the reference computes in float32: feed the oracle the float32-rounded inputs
     x32, f32, g32 = (np.float32(a).astype(np.float64) for a in (x, f, g))
