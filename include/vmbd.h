@@ -87,8 +87,9 @@ typedef struct vmbd_plan vmbd_plan;
  *   image_rows   with separable windows (vmbd_plan_set_window_factors):
  *                1 transforms each image row once per patch column and
  *                applies the patch rows' factors in the column pass, 2
- *                once per patch, 0 chooses by launch size.  The two forms
- *                round differently (each within the operators' accuracy). */
+ *                once per patch, 0 chooses (see
+ *                vmbd_plan_image_rows_shared).  The two forms round
+ *                differently (each within the operators' accuracy). */
 typedef struct vmbd_tuning {
   int32_t ola_threads;
   int32_t col_threads;
@@ -130,18 +131,22 @@ int vmbd_plan_get_tuning(const vmbd_plan* plan, vmbd_tuning* tuning);
  * overlap-add in summed form: sum_n b(c) IFFT(sum_m a(r) Y[m, n][r]) / wimg,
  * with the pixels whose covering windows are all at the floor (the outer
  * rows and columns of the patch grid) taken as the reference takes them,
- * the mean of the covering patches.  Launches large enough to fill the GPU
- * several times over also transform each image row once per patch column
- * (instead of once per covering patch) and apply the patch rows' factors
- * in the column pass.  VMBD_E_UNSUP when the tables do not factor (the
+ * the mean of the covering patches.  The plan may then also transform each
+ * image row once per patch column instead of once per covering patch,
+ * with the patch rows' factors applied in the column pass (see
+ * vmbd_plan_image_rows_shared).  VMBD_E_UNSUP when the tables do not factor (the
  * plan keeps its two-dimensional windows).  Allocates; call once after
  * vmbd_plan_create.  No counterpart in the reference's API. */
 int vmbd_plan_set_window_factors(vmbd_plan* plan, const float* a, const float* b, float floor);
 
-/* 1 when calls of nframes frames transform the image rows once per patch
- * column (separable windows, large launches), 0 when once per patch, < 0
- * on a bad argument.  The forward then runs the image and the PSF rows as
- * two launches instead of one.  No counterpart in the reference's API. */
+/* 1 when the forward and the two-adjoint call of nframes frames transform
+ * the image rows once per patch column, 0 when once per patch, < 0 on a
+ * bad argument.  Shared rows split their launch of image and PSF rows in
+ * two, so by default (image_rows 0) they take them only when the launches
+ * fill the GPU several times over.  The calls that transform the image rows
+ * alone (the f-adjoint, the image spectra, the forward of cached PSF
+ * spectra) share them whenever the windows are separable and image_rows is
+ * not 2.  No counterpart in the reference's API. */
 int vmbd_plan_image_rows_shared(const vmbd_plan* plan, int32_t nframes);
 
 /* Device bytes held by the plan (workspaces, windows, twiddles). */

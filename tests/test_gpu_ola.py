@@ -191,7 +191,10 @@ def test_batched_frames_match_single(geom, cuda):
     # cached spectra, batched
     Fs = pb.spectra('F', F)
     Xs = pb.spectra('X', X)
-    assert torch.equal(pb.forward_cached('F', Fs, X), Y)
+    # the cached forward transforms the image rows alone, once per patch
+    # column; the plain forward (image and PSF rows in one launch at this
+    # size) once per patch: the two round differently
+    assert rel_l2(pb.forward_cached('F', Fs, X).cpu().numpy(), Y.cpu().numpy()) <= 1e-6
     assert rel_l2(pb.forward_cached('X', Xs, F).cpu().numpy(), Y.cpu().numpy()) <= 1e-6
     assert torch.equal(pb.adjoint_cached('F', Fs, G, exact=True), XB)
 

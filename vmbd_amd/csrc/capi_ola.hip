@@ -228,26 +228,31 @@ struct XRows {
   bool shared;  // one transform per canvas row and patch column
 };
 
-// Per-patch image-row workgroups at or above which the shared rows are
-// used.  Below it the launches are too small to fill the GPU several times
-// over, and the column pass's factors lengthen the kernels' latency more
-// than the fewer row transforms save (1024^2 frames, 10x10 patches, tile
-// 256: 600 workgroups per frame; 1 and 2 frames run faster per patch, 4
-// and more shared).
+// Whether a call shares the image rows (separable windows; the tuning's
+// image_rows 1 forces it, 2 forbids it).  A call that transforms the image
+// rows alone shares them: fewer transforms for a small factor load in the
+// column pass.  A call that transforms image and PSF rows together (one
+// launch per patch) shares them only from XS_MIN_BLOCKS per-patch
+// image-row workgroups on, since sharing splits that launch in two: below
+// it the launches are too small to fill the GPU several times over and the
+// extra launch costs more than the fewer transforms save (1024^2 frames,
+// 10x10 patches, tile 256: 600 workgroups per frame; 1 and 2 frames run
+// faster per patch, 4 and more shared).
 constexpr long long XS_MIN_BLOCKS = 2048;
 
-bool shared_x(const vmbd_plan* p, const Geo& g) {
+bool shared_x(const vmbd_plan* p, const Geo& g, bool alone) {
   if (!p->xs || p->tune.image_rows == 2) return false;
-  if (p->tune.image_rows == 1) return true;
+  if (p->tune.image_rows == 1 || alone) return true;
   const int TPW = 256 / (g.Tw / 16);
   return (long long)g.P * (((g.sw0 + 1) / 2 + TPW - 1) / TPW) >= XS_MIN_BLOCKS;
 }
 
-XRows x_rows(const vmbd_plan* p, const Geo& g, const float* x) {
+// alone: the call transforms the image rows in a launch of their own
+XRows x_rows(const vmbd_plan* p, const Geo& g, const float* x, bool alone) {
   XRows r;
   r.g = g;
   r.s = src_image(p, x);
-  r.shared = shared_x(p, g);
+  r.shared = shared_x(p, g, alone);
   if (!r.shared) {
     r.in = raw_in(p->wsX, r.s);
     return r;
@@ -737,7 +742,7 @@ int vmbd_plan_image_rows_shared(const vmbd_plan* p, int32_t nframes) {
   if (!p) return vmbd_fail(VMBD_E_ARG, "vmbd_plan_image_rows_shared: null plan");
   if (nframes < 1 || nframes > p->max_frames)
     return vmbd_fail(VMBD_E_ARG, "nframes=%d outside [1, max_frames=%d]", nframes, p->max_frames);
-  return shared_x(p, call_geo(p, nframes)) ? 1 : 0;
+  return shared_x(p, call_geo(p, nframes), false) ? 1 : 0;
 }
 
 int vmbd_plan_get_tuning(const vmbd_plan* p, vmbd_tuning* t) {
@@ -768,7 +773,7 @@ int vmbd_ola_forward(vmbd_plan* p, int32_t nframes, const float* x, const float*
   hipStream_t st = (hipStream_t)stream;
   const Geo g = call_geo(p, nframes);
   p->rows_frames = 0;
-  const XRows xr = x_rows(p, g, x);
+  const XRows xr = x_rows(p, g, x, false);
   const RowSrc sf = src_psf(p, f);
   VMBD_LAUNCH(launch_xf_rows(p, g, xr, sf, st));
   ColOut o1{p->Y1, g.ext0}, o2{nullptr, 0};
@@ -808,7 +813,7 @@ int vmbd_ola_adjoint_f(vmbd_plan* p, int32_t nframes, const float* gres, const f
   const Geo g = call_geo(p, nframes);
   p->rows_frames = 0;
   const SharedRows sr = residual_rows(p, g, gres, exact);
-  const XRows xr = x_rows(p, g, x);
+  const XRows xr = x_rows(p, g, x, true);
   VMBD_LAUNCH(launch_rows_r2c(sr.g, sr.s, p->wsG, p->tw, st));
   VMBD_LAUNCH(launch_rows_r2c(xr.g, xr.s, p->wsX, p->tw, st, "rows_r2c_x"));
   ColOut o1{p->Y2, g.sf0}, o2{nullptr, 0};
@@ -831,7 +836,7 @@ int vmbd_ola_adjoint_xf(vmbd_plan* p, int32_t nframes, const float* gres, const 
   hipStream_t st = (hipStream_t)stream;
   const Geo g = call_geo(p, nframes);
   const SharedRows sr = residual_rows(p, g, gres, exact);
-  const XRows xr = x_rows(p, g, x);
+  const XRows xr = x_rows(p, g, x, false);
   const RowSrc sf = src_psf(p, f);
   VMBD_LAUNCH(launch_rows_r2c(sr.g, sr.s, p->wsG, p->tw, st));
   if (!reuse) {
@@ -893,7 +898,7 @@ int vmbd_spectra(vmbd_plan* p, int32_t nframes, int32_t kind, const float* z, vo
     VMBD_LAUNCH(launch_rows_r2c(g, s, p->wsF, p->tw, st, "rows_r2c_psf"));
     in = raw_in(p->wsF, s);
   } else {
-    const XRows xr = x_rows(p, g, z);
+    const XRows xr = x_rows(p, g, z, true);
     VMBD_LAUNCH(launch_rows_r2c(xr.g, xr.s, p->wsX, p->tw, st, "rows_r2c_x"));
     in = xr.in;
   }
@@ -916,7 +921,7 @@ int vmbd_ola_forward_cached_ex(vmbd_plan* p, int32_t nframes, int32_t kind, cons
   // the free operand is the image for a PSF cache and the PSFs for an image cache
   ColIn in;
   if (kind == VMBD_CACHE_PSF) {
-    const XRows xr = x_rows(p, g, u);
+    const XRows xr = x_rows(p, g, u, true);
     VMBD_LAUNCH(launch_rows_r2c(xr.g, xr.s, p->wsX, p->tw, st, "rows_r2c_x"));
     in = xr.in;
   } else {
@@ -1187,7 +1192,7 @@ BatchGroup batch_group(vmbd_plan* p, int nframes, const float* x, const float* f
   const Geo g = call_geo(p, nframes);
   B.g = g;
   B.tw = p->tw;
-  const XRows xr = x_rows(p, g, x);
+  const XRows xr = x_rows(p, g, x, false);
   const RowSrc sf = src_psf(p, f);
   const int TPW = WG / (g.Tw / 16);
   auto blocks = [&](const RowSrc& s) { return ((s.rhi - s.rlo + 1) / 2 + TPW - 1) / TPW; };
