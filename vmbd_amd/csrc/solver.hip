@@ -184,18 +184,23 @@ __device__ __forceinline__ float pbb_new(float xv, float gv, float step, int unc
   return (!unconstrained && v < 0.f) ? 0.f : v;
 }
 
-// n4 as in k_pbb_stats; the operands are loaded before the partials are
-// reduced, so their latency overlaps the reduction.
+// n4 as in k_pbb_stats; the first two chunks of each thread are loaded
+// before the partials are reduced, so their latency overlaps the reduction.
 __global__ __launch_bounds__(TB) void k_pbb_update(float* xnew, const float* x, const float* g, long long n,
                                                    long long n4, const double* partials, int nb, int iter,
                                                    int unconstrained, float* step_out) {
   const long long stride = (long long)gridDim.x * TB;
   const long long tid = (long long)blockIdx.x * TB + threadIdx.x;
   const float4 *x4 = (const float4*)x, *g4 = (const float4*)g;
-  float4 a = make_float4(0.f, 0.f, 0.f, 0.f), c = a;
+  const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+  float4 a0 = z4, c0 = z4, a1 = z4, c1 = z4;
   if (tid < n4) {
-    a = x4[tid];
-    c = g4[tid];
+    a0 = x4[tid];
+    c0 = g4[tid];
+  }
+  if (tid + stride < n4) {
+    a1 = x4[tid + stride];
+    c1 = g4[tid + stride];
   }
   double s[3];
   reduce_partials<3>(partials, nb, s);
@@ -204,10 +209,9 @@ __global__ __launch_bounds__(TB) void k_pbb_update(float* xnew, const float* x, 
   if (blockIdx.x == 0 && threadIdx.x == 0 && step_out) *step_out = step;
   float4* o4 = (float4*)xnew;
   for (long long e = tid; e < n4; e += stride) {
-    if (e != tid) {
-      a = x4[e];
-      c = g4[e];
-    }
+    // the first two chunks were loaded before the reduction
+    const float4 a = e == tid ? a0 : e == tid + stride ? a1 : x4[e];
+    const float4 c = e == tid ? c0 : e == tid + stride ? c1 : g4[e];
     o4[e] = make_float4(pbb_new(a.x, c.x, step, unconstrained), pbb_new(a.y, c.y, step, unconstrained),
                         pbb_new(a.z, c.z, step, unconstrained), pbb_new(a.w, c.w, step, unconstrained));
   }
@@ -416,6 +420,16 @@ inline int grid_for(long long n) {
   if (b > VMBD_MAX_PARTIALS) b = VMBD_MAX_PARTIALS;
   return b < 1 ? 1 : (int)b;
 }
+
+// blocks of the PBB statistics and update passes: two 4-float chunks per
+// thread (so a small operand, such as a PSF stack, runs few blocks and its
+// update reduces few partials), at most VMBD_MAX_PARTIALS; never more than
+// grid_for(n), the partials callers allocate
+inline int grid_pbb(long long n) {
+  long long b = (n + 8 * TB - 1) / (8 * TB);
+  if (b > VMBD_MAX_PARTIALS) b = VMBD_MAX_PARTIALS;
+  return b < 1 ? 1 : (int)b;
+}
 #define SOLVER_DONE(what)                                            \
   do {                                                               \
     hipError_t e__ = hipGetLastError();                              \
@@ -463,7 +477,7 @@ int vmbd_pbb_stats(const float* x, const float* oldx, const float* g, const floa
   vmbd_clear_error();
   if (!x || !oldx || !g || !oldg || !partials || n <= 0) return vmbd_fail(VMBD_E_ARG, "vmbd_pbb_stats: bad argument");
   hipStream_t st = (hipStream_t)stream;
-  const int nb = grid_for(n);
+  const int nb = grid_pbb(n);
   ProfScope ps("pbb_stats", st);
   const long long n4 = vec4_chunks(n, {x, oldx, g, oldg});
   hipLaunchKernelGGL(k_pbb_stats, dim3(nb), dim3(TB), 0, st, x, oldx, g, oldg, (long long)n, n4, unconstrained,
@@ -477,7 +491,7 @@ int vmbd_pbb_update(float* xnew, const float* x, const float* g, int64_t n, cons
   vmbd_clear_error();
   if (!xnew || !x || !g || !partials || n <= 0) return vmbd_fail(VMBD_E_ARG, "vmbd_pbb_update: bad argument");
   hipStream_t st = (hipStream_t)stream;
-  const int nb = grid_for(n);
+  const int nb = grid_pbb(n);
   ProfScope ps("pbb_update", st);
   const long long n4 = vec4_chunks(n, {xnew, x, g});
   hipLaunchKernelGGL(k_pbb_update, dim3(nb), dim3(TB), 0, st, xnew, x, g, (long long)n, n4, partials, nb, iter,
@@ -491,7 +505,7 @@ int vmbd_pbb_step(float* xnew, const float* x, const float* oldx, const float* g
   if (!xnew || !x || !oldx || !g || !oldg || !partials || n <= 0)
     return vmbd_fail(VMBD_E_ARG, "vmbd_pbb_step: bad argument");
   hipStream_t st = (hipStream_t)stream;
-  const int nb = grid_for(n);
+  const int nb = grid_pbb(n);
   ProfScope ps("pbb_step", st);
   const long long n4s = vec4_chunks(n, {x, oldx, g, oldg});
   const long long n4u = vec4_chunks(n, {xnew, x, g});
