@@ -143,10 +143,10 @@ int vmbd_plan_set_window_factors(vmbd_plan* plan, const float* a, const float* b
  * the image rows once per patch column, 0 when once per patch, < 0 on a
  * bad argument.  Shared rows split their launch of image and PSF rows in
  * two, so by default (image_rows 0) they take them only when the launches
- * fill the GPU several times over.  The calls that transform the image rows
- * alone (the f-adjoint, the image spectra, the forward of cached PSF
- * spectra) share them whenever the windows are separable and image_rows is
- * not 2.  No counterpart in the reference's API. */
+ * fill the GPU several times over; vmbd_ola_adjoint_f follows them.  The
+ * calls that transform the image rows alone (the image spectra, the forward
+ * of cached PSF spectra) share them whenever the windows are separable and
+ * image_rows is not 2.  No counterpart in the reference's API. */
 int vmbd_plan_image_rows_shared(const vmbd_plan* plan, int32_t nframes);
 
 /* Device bytes held by the plan (workspaces, windows, twiddles). */

@@ -230,8 +230,8 @@ struct XRows {
 
 // Whether a call shares the image rows (separable windows; the tuning's
 // image_rows 1 forces it, 2 forbids it).  A call that transforms the image
-// rows alone shares them: fewer transforms for a small factor load in the
-// column pass.  A call that transforms image and PSF rows together (one
+// rows alone (the image spectra, the forward of cached PSF spectra) shares
+// them: fewer transforms for a small factor load in the column pass.  A call that transforms image and PSF rows together (one
 // launch per patch) shares them only from XS_MIN_BLOCKS per-patch
 // image-row workgroups on, since sharing splits that launch in two: below
 // it the launches are too small to fill the GPU several times over and the
@@ -813,7 +813,8 @@ int vmbd_ola_adjoint_f(vmbd_plan* p, int32_t nframes, const float* gres, const f
   const Geo g = call_geo(p, nframes);
   p->rows_frames = 0;
   const SharedRows sr = residual_rows(p, g, gres, exact);
-  const XRows xr = x_rows(p, g, x, true);
+  // the form of vmbd_ola_adjoint_xf, so that both give f-bar bit for bit
+  const XRows xr = x_rows(p, g, x, false);
   VMBD_LAUNCH(launch_rows_r2c(sr.g, sr.s, p->wsG, p->tw, st));
   VMBD_LAUNCH(launch_rows_r2c(xr.g, xr.s, p->wsX, p->tw, st, "rows_r2c_x"));
   ColOut o1{p->Y2, g.sf0}, o2{nullptr, 0};

@@ -93,10 +93,11 @@ class OlaPlan:
         """True when the forward and adjoint_xf of `nframes` frames transform
         each image row once per patch column (separable windows, and by
         default launches large enough to fill the GPU several times over);
-        the forward then runs its image and PSF rows as two launches.  The
-        calls that transform the image rows alone (adjoint_f, the image
-        spectra, forward_cached with PSF spectra) share them whenever the
-        windows are separable and image_rows is not 2."""
+        the forward then runs its image and PSF rows as two launches, and
+        adjoint_f follows the same choice.  The calls that transform the
+        image rows alone (the image spectra, forward_cached with PSF spectra)
+        share them whenever the windows are separable and image_rows is not
+        2."""
         rc = _lib.lib().vmbd_plan_image_rows_shared(self._h, int(nframes))
         if rc < 0:
             _lib.check(rc, 'vmbd_plan_image_rows_shared')
