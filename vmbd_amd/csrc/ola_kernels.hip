@@ -1276,7 +1276,27 @@ void ola_layout(const Geo& g, OlaDst& d) {
   d.stream = ola_stream_rounds(g, d);
   d.ncls = 0;
   d.sumr = 0;
-  if (!d.alias || g.hop1 <= 0) return;
+  if (g.hop1 <= 0) return;
+  if (!d.alias) {
+    // the per-patch transforms of a row take several rounds; the summed form
+    // needs one transform per pair of patch columns, however many patch
+    // rows cover the row, so it runs in one round whenever those fit a
+    // workgroup (and, with windows, they factor)
+    const int NP2 = (g.csf1 + 1) / 2;
+    const int TPW = WG / (g.Tw / 16);
+    if ((d.win && !d.wrow) || NP2 > TPW) return;
+    OlaDst f = d;
+    f.ncls = std::min(g.csf1, (d.ext1 + g.hop1 - 1) / g.hop1);
+    for (int R = TPW / NP2; R >= 1; --R) {
+      f.sumr = R;
+      if (ola_shared_bytes(g, f) <= (size_t)160 * 1024) {
+        d.ncls = f.ncls;
+        d.sumr = R;
+        return;
+      }
+    }
+    return;
+  }
   OlaDst e = d;
   e.ncls = std::min(g.csf1, (d.ext1 + g.hop1 - 1) / g.hop1);
   const size_t b0 = ola_shared_bytes(g, d), b1 = ola_shared_bytes(g, e);
