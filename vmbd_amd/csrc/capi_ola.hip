@@ -231,13 +231,14 @@ struct XRows {
 // Whether a call shares the image rows (separable windows; the tuning's
 // image_rows 1 forces it, 2 forbids it).  A call that transforms the image
 // rows alone (the image spectra, the forward of cached PSF spectra) shares
-// them: fewer transforms for a small factor load in the column pass.  A call that transforms image and PSF rows together (one
-// launch per patch) shares them only from XS_MIN_BLOCKS per-patch
-// image-row workgroups on, since sharing splits that launch in two: below
-// it the launches are too small to fill the GPU several times over and the
-// extra launch costs more than the fewer transforms save (1024^2 frames,
-// 10x10 patches, tile 256: 600 workgroups per frame; 1 and 2 frames run
-// faster per patch, 4 and more shared).
+// them: fewer transforms for a small factor load in the column pass.  A
+// call that transforms image and PSF rows together (one launch per patch),
+// and the f-adjoint, which must match it bit for bit, share them only from
+// XS_MIN_BLOCKS per-patch image-row workgroups on, since sharing splits
+// that launch in two: below it the launches are too small to fill the GPU
+// several times over and the extra launch costs more than the fewer
+// transforms save (1024^2 frames, 10x10 patches, tile 256: 600 workgroups
+// per frame; 1 and 2 frames run faster per patch, 4 and more shared).
 constexpr long long XS_MIN_BLOCKS = 2048;
 
 bool shared_x(const vmbd_plan* p, const Geo& g, bool alone) {
