@@ -87,6 +87,10 @@ def kernel_models(plan, B):
         'rows_r2c_x': dict(bytes=B * (H * W * 4 + plan.csf[1] * NB * sx0 * 8),
                            flops=B * plan.csf[1] * half(sx0) * rf),
         'rows_r2c_psf': dict(bytes=B * (P * sf0 * sf1 * 4 + P * NB * sf0 * 8), flops=B * P * half(sf0) * rf),
+        # the shared image rows and the PSF rows in one launch
+        'rows_r2c_x_psf': dict(bytes=B * (H * W * 4 + plan.csf[1] * NB * sx0 * 8 + P * sf0 * sf1 * 4
+                                          + P * NB * sf0 * 8),
+                               flops=B * (plan.csf[1] * half(sx0) + P * half(sf0)) * rf),
         # residual rows shared by the patches of a patch column: one transform
         # per (patch column, virtual row pair) over (csf0-1)*hop0 + ext0 rows
         'rows_r2c': dict(bytes=B * (sy0 * sy1 * 4 + plan.csf[1] * NB * nv * 8),

@@ -141,9 +141,10 @@ int vmbd_plan_set_window_factors(vmbd_plan* plan, const float* a, const float* b
 
 /* 1 when the forward and the two-adjoint call of nframes frames transform
  * the image rows once per patch column, 0 when once per patch, < 0 on a
- * bad argument.  Shared rows split their launch of image and PSF rows in
- * two, so by default (image_rows 0) they take them only when the launches
- * fill the GPU several times over; vmbd_ola_adjoint_f follows them.  The
+ * bad argument.  By default (image_rows 0) they share them only when the
+ * launches fill the GPU (a single 1024^2 frame does not); the factor load
+ * of the column pass costs more than it saves in launches that leave most
+ * of it idle.  vmbd_ola_adjoint_f follows them.  The
  * calls that transform the image rows alone (the image spectra, the forward
  * of cached PSF spectra) share them whenever the windows are separable and
  * image_rows is not 2.  No counterpart in the reference's API. */

@@ -18,10 +18,9 @@ launch forms:
   the groups are split by tile into one batch per tile and ``capture()``
   forks those batches over k streams (24 launches on three streams).
   Groups whose image rows are shared per patch column (separable windows,
-  the default, at launches that fill the GPU several times over) transform
-  them in a row pass of their own.
-* ``grouped=False``: the groups' own launches, seven per group (eight with
-  shared image rows); with
+  the default, at launches that fill the GPU) transform them in a row pass
+  of their own.
+* ``grouped=False``: the groups' own launches, seven per group; with
   ``streams=k > 1`` (default 3) ``capture()`` forks the groups over k
   streams inside the graph (largest groups first, each to the stream with
   the fewest pixels so far) so that one group's launches fill the tail of
@@ -139,7 +138,7 @@ class MixedBatch:
         """Kernel launches per step."""
         if self.grouped:
             return sum(int(_lib.lib().vmbd_batch_launches(vb)) for vb in self._batches().values())
-        return sum(7 + int(G['plan'].image_rows_shared(len(G['idx']))) for G in self.groups)
+        return 7 * len(self.groups)
 
     def _batch_step(self, vb):
         _lib.check(_lib.lib().vmbd_batch_step(vb, _lib.stream_ptr()), 'vmbd_batch_step')

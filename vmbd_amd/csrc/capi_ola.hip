@@ -232,14 +232,14 @@ struct XRows {
 // image_rows 1 forces it, 2 forbids it).  A call that transforms the image
 // rows alone (the image spectra, the forward of cached PSF spectra) shares
 // them: fewer transforms for a small factor load in the column pass.  A
-// call that transforms image and PSF rows together (one launch per patch),
-// and the f-adjoint, which must match it bit for bit, share them only from
-// XS_MIN_BLOCKS per-patch image-row workgroups on, since sharing splits
-// that launch in two: below it the launches are too small to fill the GPU
-// several times over and the extra launch costs more than the fewer
-// transforms save (1024^2 frames, 10x10 patches, tile 256: 600 workgroups
-// per frame; 1 and 2 frames run faster per patch, 4 and more shared).
-constexpr long long XS_MIN_BLOCKS = 2048;
+// call that transforms image and PSF rows together, and the f-adjoint,
+// which must match it bit for bit, shares them only from XS_MIN_BLOCKS
+// per-patch image-row workgroups on: in smaller launches, which leave most
+// of the GPU idle, the column pass's factor load lengthens the kernels more
+// than the fewer row transforms save (1024^2 frames, 10x10 patches, tile
+// 256: 600 workgroups per frame; 1 frame runs faster per patch, 2 and more
+// shared).
+constexpr long long XS_MIN_BLOCKS = 1024;
 
 bool shared_x(const vmbd_plan* p, const Geo& g, bool alone) {
   if (!p->xs || p->tune.image_rows == 2) return false;
@@ -340,17 +340,19 @@ OlaDst dst_adjoint_x(const vmbd_plan* p, float* xbar) {
   return d;
 }
 
-// row transforms of x and f: one launch over both (one geometry), or, with
-// the image rows shared per patch column, one launch each
+// row transforms of x and f: one launch over both (one geometry, or, with
+// the image rows shared per patch column, a geometry per job)
 hipError_t launch_xf_rows(const vmbd_plan* p, const Geo& g, const XRows& xr, const RowSrc& sf, hipStream_t st) {
   if (!xr.shared) {
     RowSrc srcs[2] = {xr.s, sf};
     float2* outs[2] = {p->wsX, p->wsF};
     return launch_rows_r2c_multi(g, srcs, outs, 2, p->tw, st);
   }
-  hipError_t e = launch_rows_r2c(xr.g, xr.s, p->wsX, p->tw, st, "rows_r2c_x");
-  if (e == hipSuccess) e = launch_rows_r2c(g, sf, p->wsF, p->tw, st, "rows_r2c_psf");
-  return e;
+  // the shared image rows (one geometry per patch column) beside the PSF rows
+  const RowSrc srcs[2] = {xr.s, sf};
+  float2* outs[2] = {p->wsX, p->wsF};
+  const int P[2] = {xr.g.P, g.P}, Pf[2] = {xr.g.Pf, g.Pf};
+  return launch_rows_r2c_geo(g, srcs, outs, P, Pf, 2, p->tw, st, "rows_r2c_x_psf");
 }
 
 #define VMBD_LAUNCH(expr)                                            \

@@ -181,6 +181,16 @@ struct DivParams {
 };
 
 // Launchers (ola_kernels.hip).  Return hipError_t of the launch.
+// Jobs of different patch counts in one launch (the shared image rows of a
+// patch column beside the PSF rows of every patch): job j has P[j] units of
+// its own geometry's patches (Pf[j] per frame); at most two jobs.
+struct RowJobGeo {
+  int P[2], Pf[2];
+  int nb[2];    // row blocks per patch
+  int uend[2];  // cumulative work units, P[j] * nb[j]
+};
+hipError_t launch_rows_r2c_geo(const Geo& g, const RowSrc* s, float2* const* out, const int* P, const int* Pf, int n,
+                               const float2* tw, hipStream_t st, const char* name);
 // name: the launch's label in the library's kernel timing (default rows_r2c)
 hipError_t launch_rows_r2c(const Geo& g, const RowSrc& s, float2* out, const float2* tw, hipStream_t st,
                            const char* name = nullptr);

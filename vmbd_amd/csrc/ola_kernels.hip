@@ -27,6 +27,7 @@
 // the transposed store alias it (separated by workgroup barriers), which keeps
 // the footprint at 35-40 KB and allows four workgroups per CU.
 #include <algorithm>
+#include <climits>
 
 #include "col_device.h"
 #include "fft_core.h"
@@ -254,6 +255,71 @@ __global__ __launch_bounds__(WG) void k_rows_r2c(Geo g, RowJobs jobs, const floa
   rows_unit<TW>(g, jobs, (int)blockIdx.x, pu, ji, q0);
   const RowSrc s = jobs.s[ji];
   rows_r2c_body<TW>(g, s, jobs.out[ji], pu, q0, sm, tw);
+}
+
+// Row jobs with a patch count of their own: the units of job 0, then of
+// job 1, each in the tile order of rows_unit over its own patches; the XCD
+// of workgroup b runs a contiguous range of them.
+template <int TW>
+__global__ __launch_bounds__(WG) void k_rows_r2c_geo(Geo g, RowJobs jobs, RowJobGeo jg,
+                                                     const float2* __restrict__ tw) {
+  __shared__ float2 sm[RowsLds<TW>::v];
+  constexpr int TPW = WG / (TW / 16);
+  const int blk = (int)blockIdx.x;
+  const int total = jobs.n > 1 ? jg.uend[1] : jg.uend[0];
+  const int xcd = blk % NXCD;
+  const int u = xcd * (total / NXCD) + min(xcd, total % NXCD) + blk / NXCD;
+  const bool j1 = jobs.n > 1 && u >= jg.uend[0];
+  const int ul = j1 ? u - jg.uend[0] : u;
+  const int P = j1 ? jg.P[1] : jg.P[0];
+  const int nb = j1 ? jg.nb[1] : jg.nb[0];
+  const int TP = (P + NXCD - 1) / NXCD;
+  const int tile = ul / (TP * nb);
+  const int ur = ul - tile * TP * nb;
+  const int Tt = min(TP, P - tile * TP);
+  const int by = ur / Tt;
+  const int pu = tile * TP + (ur - by * Tt);
+  Geo gj = g;
+  gj.P = P;
+  gj.Pf = j1 ? jg.Pf[1] : jg.Pf[0];
+  rows_r2c_body<TW>(gj, j1 ? jobs.s[1] : jobs.s[0], j1 ? jobs.out[1] : jobs.out[0], pu, by * TPW, sm, tw);
+}
+
+template <int TW>
+static hipError_t rows_r2c_geo_t(const Geo& g, const RowSrc* s, float2* const* out, const int* P, const int* Pf, int n,
+                                 const float2* tw, hipStream_t st, const char* name) {
+  constexpr int TPW = WG / (TW / 16);
+  RowJobs jobs{};
+  RowJobGeo jg{};
+  jobs.n = n;
+  long long units = 0;
+  for (int i = 0; i < n; ++i) {
+    jobs.s[i] = s[i];
+    jobs.out[i] = out[i];
+    jg.P[i] = P[i];
+    jg.Pf[i] = Pf[i];
+    jg.nb[i] = ((s[i].rhi - s[i].rlo + 1) / 2 + TPW - 1) / TPW;
+    units += (long long)P[i] * jg.nb[i];
+    if (jg.nb[i] <= 0 || P[i] <= 0 || units > INT_MAX) return hipErrorInvalidValue;
+    jg.uend[i] = (int)units;
+  }
+  ProfScope ps(name ? name : "rows_r2c_multi", st);
+  hipLaunchKernelGGL(k_rows_r2c_geo<TW>, dim3((unsigned)units), dim3(WG), 0, st, g, jobs, jg, tw);
+  return hipGetLastError();
+}
+
+hipError_t launch_rows_r2c_geo(const Geo& g, const RowSrc* s, float2* const* out, const int* P, const int* Pf, int n,
+                               const float2* tw, hipStream_t st, const char* name) {
+  if (n < 1 || n > 2) return hipErrorInvalidValue;
+  switch (g.Tw) {
+    case 16: return rows_r2c_geo_t<16>(g, s, out, P, Pf, n, tw, st, name);
+    case 32: return rows_r2c_geo_t<32>(g, s, out, P, Pf, n, tw, st, name);
+    case 64: return rows_r2c_geo_t<64>(g, s, out, P, Pf, n, tw, st, name);
+    case 128: return rows_r2c_geo_t<128>(g, s, out, P, Pf, n, tw, st, name);
+    case 256: return rows_r2c_geo_t<256>(g, s, out, P, Pf, n, tw, st, name);
+    case 512: return rows_r2c_geo_t<512>(g, s, out, P, Pf, n, tw, st, name);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 template <int TW>

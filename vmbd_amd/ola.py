@@ -92,9 +92,8 @@ class OlaPlan:
     def image_rows_shared(self, nframes=1):
         """True when the forward and adjoint_xf of `nframes` frames transform
         each image row once per patch column (separable windows, and by
-        default launches large enough to fill the GPU several times over);
-        the forward then runs its image and PSF rows as two launches, and
-        adjoint_f follows the same choice.  The calls that transform the
+        default launches large enough to fill the GPU; a single 1024^2 frame
+        is not), and adjoint_f follows the same choice.  The calls that transform the
         image rows alone (the image spectra, forward_cached with PSF spectra)
         share them whenever the windows are separable and image_rows is not
         2."""
