@@ -14,7 +14,7 @@ launch forms:
   launch per pass and distinct tile (the row, overlap-add and stack passes
   are specialised to the tile width, the column passes to its height).  With
   ``streams=1`` one batch holds every group (64 frames of six geometries on
-  three tiles: 22 launches per step instead of 47); with ``streams=k > 1``
+  three tiles: 22 launches per step instead of 42); with ``streams=k > 1``
   the groups are split by tile into one batch per tile and ``capture()``
   forks those batches over k streams (24 launches on three streams).
   Groups whose image rows are shared per patch column (separable windows,
