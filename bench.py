@@ -4,11 +4,13 @@
 Workload (one "step"): forward y = B(x, f) plus both exact transposes
 (x-adjoint F'g and f-adjoint X'g of a residual g) for a batch of frames of
 1024 x 1024 float32 images, 10 x 10 patches at 50 % overlap (window 188,
-hop 94, FFT tile 256 x 256), 41 x 41 motion-streak PSFs.  All patches of all
-frames of the batch go through the same launches; the step is captured once
-in a HIP graph and replayed.  Inputs are synthetic (seeded piecewise-smooth
-images, random-walk PSFs, Gaussian residuals) and already resident in HBM
-when the timed region starts.
+hop 94, FFT tile 256 x 256), 41 x 41 motion-streak PSFs.  The step holds
+`streams` batches of `frames` frames (default 2 x 32), each a plan of its
+own whose launches carry all patches of its frames, the batches forked over
+that many streams; the step is captured once in a HIP graph and replayed.
+Inputs are synthetic (seeded piecewise-smooth images, random-walk PSFs,
+Gaussian residuals), distinct per frame, and already resident in HBM when
+the timed region starts.
 
 Prints ONE JSON line.  value = Mpix/s = frames * 1024^2 / time per step.
 Beside it: the same step in the plain per-operator GPU form (the
@@ -16,7 +18,7 @@ reference's launch sequence, one kernel per operation, full complex spectra)
 and the online MFBD loop in frames/s (1024^2 latent, 20 projected-BB
 f-iterations + 20 x-iterations per frame, graph-captured).
 
-    python bench.py [--steps K] [--warmup W] [--frames B] [--no-graph] [--cpu-seconds S]
+    python bench.py [--steps K] [--warmup W] [--frames B] [--streams S] [--no-graph] [--cpu-seconds S]
                     [--no-plain] [--no-mfbd] [--mfbd-frames N] [--no-mixed] [--no-deconv]
 """
 import argparse
@@ -40,7 +42,9 @@ def parse():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=50)
     ap.add_argument('--warmup', type=int, default=5)
-    ap.add_argument('--frames', type=int, default=32, help='frames per step, batched into the same launches')
+    ap.add_argument('--frames', type=int, default=32, help='frames per batch, batched into the same launches')
+    ap.add_argument('--streams', type=int, default=2,
+                    help='batches per step, each on a stream of its own inside the graph')
     ap.add_argument('--no-graph', action='store_true', help='launch eagerly instead of replaying a HIP graph')
     ap.add_argument('--cpu-seconds', type=float, default=18.0,
                     help='bounded CPU-baseline sample (split over its three variants)')
@@ -132,21 +136,22 @@ def attainable_hbm_frac():
     return (operator_bytes() / (HBM_PEAK_GBS * 1e9)) / (operator_flops() / (VALU_FP32_PEAK * 1e12))
 
 
-def operator_roofline(kernels, B, ms):
+def operator_roofline(kernels, B, ms, nstep):
     """Roofline on the operator's algorithmic bytes and flops (one fwd +
     x-adj + f-adj per frame, inputs and outputs once, windows generated in
     the kernels): `achieved`/`frac` are the B frames' bytes over the dominant
-    kernel's average launch time (the intermediate spectra it moves are not
-    algorithmic bytes; they are under `kernels`), `step` the same bytes and
-    the flops over the whole step's time.  The step is VALU-issue bound (52
+    kernel's average launch time (a launch carries B frames; the intermediate
+    spectra it moves are not algorithmic bytes, they are under `kernels`),
+    `step` the nstep frames' bytes and flops over the whole step's time.  The step is VALU-issue bound (52
     flop per byte against a ridge of 10 for non-packed FP32), so `valu`
     gives its flop rate against the non-packed FP32 peak, and the HBM
     fraction it can reach is capped at attainable_hbm_frac()."""
     dom = max(kernels, key=lambda k: kernels[k]['total_us_per_step'])
     d = kernels[dom]
-    by, fl = operator_bytes() * B, operator_flops() * B
+    by = operator_bytes() * B
     gbs = by / (d['avg_us'] * 1e-6) / 1e9
-    step_gbs = by / (ms * 1e-3) / 1e9
+    step_gbs = operator_bytes() * nstep / (ms * 1e-3) / 1e9
+    fl = operator_flops() * nstep
     step_tf = fl / (ms * 1e-3) / 1e12
     return dict(kernel=dom, bound='valu', achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit='GB/s',
                 frac=round(gbs / HBM_PEAK_GBS, 4), traffic=measured_traffic(dom, B), avg_us=d['avg_us'],
@@ -346,18 +351,25 @@ def main():
 
     dev = torch.device('cuda', 0)
     torch.cuda.set_device(dev)
-    B = a.frames
+    B, S = a.frames, a.streams
+    if B < 1 or S < 1:
+        sys.exit('bench.py: --frames and --streams take positive counts')
+    N = S * B  # frames per step
     wa = win2winaux((H, W), CSF, OVERLAP)
-    plan = OlaPlan((H, W), SF, wa, mode='valid', device=dev, max_frames=B)
+    plans = [OlaPlan((H, W), SF, wa, mode='valid', device=dev, max_frames=B) for _ in range(S)]
+    plan = plans[0]
     rng = np.random.default_rng(1234)
-    x_np = np.stack([synth.piecewise_smooth((H, W), rng) for _ in range(B)])
-    f_np = np.stack([synth.motion_psfs(plan.P, SF, rng, steps=40) for _ in range(B)])
-    g_np = rng.standard_normal((B,) + plan.sy) * 0.01
+    x_np = np.stack([synth.piecewise_smooth((H, W), rng) for _ in range(N)])
+    f_np = np.stack([synth.motion_psfs(plan.P, SF, rng, steps=40) for _ in range(N)])
+    g_np = rng.standard_normal((N,) + plan.sy) * 0.01
     to = lambda v: torch.as_tensor(np.ascontiguousarray(v, dtype=np.float32), device=dev)
     x, f, g = to(x_np), to(f_np), to(g_np)
-    y = torch.empty((B,) + plan.sy, device=dev)
-    xb = torch.empty((B, H, W), device=dev)
-    fb = torch.empty((B, plan.P) + SF, device=dev)
+    y = torch.empty((N,) + plan.sy, device=dev)
+    xb = torch.empty((N, H, W), device=dev)
+    fb = torch.empty((N, plan.P) + SF, device=dev)
+    # batch k's views, made once: the adjoint reuses the forward's row
+    # transforms only for the same tensors
+    views = [[t[k * B:(k + 1) * B] for t in (x, f, g, y, xb, fb)] for k in range(S)]
 
     if a.calib:
         src = torch.ones(16 * 1024 * 1024, device=dev)
@@ -366,9 +378,26 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    def step():
-        plan.forward(x, f, out=y)
-        plan.adjoint_xf(g, x, f, exact=True, out_x=xb, out_f=fb)
+    def batch(k):
+        xk, fk, gk, yk, xbk, fbk = views[k]
+        plans[k].forward(xk, fk, out=yk)
+        plans[k].adjoint_xf(gk, xk, fk, exact=True, out_x=xbk, out_f=fbk)
+
+    def step():  # eager: the batches one after the other on the current stream
+        for k in range(S):
+            batch(k)
+
+    def forked_step(side):
+        """The batches over the current stream and the streams in side,
+        forked from and joined back into the current one."""
+        cur = torch.cuda.current_stream()
+        for ss in side:
+            ss.wait_stream(cur)
+        for k in range(S):
+            with torch.cuda.stream(cur if k == 0 else side[k - 1]):
+                batch(k)
+        for ss in side:
+            cur.wait_stream(ss)
 
     def timed(fn, steps):
         torch.cuda.synchronize()
@@ -389,17 +418,18 @@ def main():
     graph = None
     if not a.no_graph:
         s = torch.cuda.Stream()
+        side = [torch.cuda.Stream() for _ in range(S - 1)]
         s.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(s):
             step()  # the adjoint's reuse decision is taken outside the capture
             graph = torch.cuda.CUDAGraph()
             with torch.cuda.graph(graph, stream=s):
-                step()
+                forked_step(side)
         torch.cuda.current_stream().wait_stream(s)
         for _ in range(a.warmup):
             graph.replay()
         ms = timed(graph.replay, a.steps)
-    mpix = B * H * W / (ms * 1e-3) / 1e6
+    mpix = N * H * W / (ms * 1e-3) / 1e6
 
     # single-frame eager step, for reference
     plan1 = OlaPlan((H, W), SF, wa, mode='valid', device=dev, max_frames=1)
@@ -434,8 +464,8 @@ def main():
             e['nominal_flops'] = mm['flops']
             e['tflops'] = round(mm['flops'] / (avg * 1e-6) / 1e12, 3)
         kernels[name] = e
-    roofline = operator_roofline(kernels, B, ms)
-    op_bytes = operator_bytes() * B
+    roofline = operator_roofline(kernels, B, ms, N)
+    op_bytes = operator_bytes() * N
 
     plain = None
     if not a.no_plain:
@@ -525,8 +555,9 @@ def main():
         'data': 'synthetic (seeded piecewise-smooth 1024^2 images, 41x41 random-walk motion PSFs, N(0,0.01^2) residual)',
         'config': {'workload': 'EFF/OLA forward + exact x-adjoint + exact f-adjoint, 1024x1024 fp32, '
                                '10x10 patches 50% overlap (sw 188, hop 94), 41x41 PSFs, FFT tile 256x256',
-                   'frames_per_step': B, 'graph': graph is not None, 'mode': 'valid'},
-        'ms_per_frame': round(ms / B, 4),
+                   'frames_per_step': N, 'frames_per_launch': B, 'streams': S, 'graph': graph is not None,
+                   'mode': 'valid'},
+        'ms_per_frame': round(ms / N, 4),
         'eager_batched_ms_per_step': round(eager_ms, 4),
         'single_frame_eager': {'ms': round(ms1, 4), 'mpix_s': round(H * W / (ms1 * 1e-3) / 1e6, 2)},
         'step_alg_bytes': op_bytes,
