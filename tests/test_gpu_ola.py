@@ -284,6 +284,58 @@ def test_graph_capture_replay(cuda):
         _ = (xt * 2).sum().item()  # unrelated eager work between replays
 
 
+@pytest.mark.parametrize('image_rows', [0, 1], ids=['rows-by-size', 'rows-shared'])
+def test_graph_two_streams(image_rows, cuda):
+    """Two plans of three frames each, forked over two streams inside one
+    graph (the benchmark's step): every replay equals each plan's eager
+    results bit for bit, so the plans share no device state."""
+    import torch
+    from vmbd_amd.imagetools import win2winaux
+    from vmbd_amd.ola import OlaPlan
+    print(f'case two plans on two streams in one graph, image_rows={image_rows}', flush=True)
+    sx, csf, sf, B = (256, 256), (4, 4), (15, 15), 3
+    wa = win2winaux(sx, csf, 0.5)
+    plans = [OlaPlan(sx, sf, wa, mode='valid', device=cuda, max_frames=B) for _ in range(2)]
+    for p in plans:
+        p.set_tuning(image_rows=image_rows)
+    cases = [case_inputs(40 + i, sx, csf, sf) for i in range(2 * B)]
+    x, f, g = (_t(np.stack([c[j] for c in cases]), cuda) for j in range(3))
+    y = torch.empty((2 * B,) + plans[0].sy, device=cuda)
+    xb = torch.empty((2 * B,) + sx, device=cuda)
+    fb = torch.empty((2 * B, 16) + sf, device=cuda)
+    views = [[t[k * B:(k + 1) * B] for t in (x, f, g, y, xb, fb)] for k in range(2)]
+
+    def batch(k):
+        xk, fk, gk, yk, xbk, fbk = views[k]
+        plans[k].forward(xk, fk, out=yk)
+        plans[k].adjoint_xf(gk, xk, fk, exact=True, out_x=xbk, out_f=fbk)
+
+    for k in range(2):
+        batch(k)
+    torch.cuda.synchronize()
+    e = (y.clone(), xb.clone(), fb.clone())
+    s, side = torch.cuda.Stream(), torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        for k in range(2):
+            batch(k)
+        gr = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(gr, stream=s):
+            cur = torch.cuda.current_stream()
+            side.wait_stream(cur)
+            batch(0)
+            with torch.cuda.stream(side):
+                batch(1)
+            cur.wait_stream(side)
+    torch.cuda.current_stream().wait_stream(s)
+    for it in range(3):
+        y.zero_(); xb.zero_(); fb.zero_()
+        gr.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(y, e[0]) and torch.equal(xb, e[1]) and torch.equal(fb, e[2]), it
+        _ = (x * 2).sum().item()  # unrelated eager work between replays
+
+
 def test_named_entry_points(cuda):
     """vmbd_windows expands the window classes to win2winaux's ws exactly;
     vmbd_cache_{psf,image}_spectra equal vmbd_spectra; vmbd_pbb_step equals
