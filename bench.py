@@ -342,6 +342,9 @@ def main():
         print(f'bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU sharding of this '
               'operator is a design note, not a measured path.', file=sys.stderr)
         sys.exit(2)
+    if a.frames < 1 or a.streams < 1:
+        print('bench.py: --frames and --streams take positive counts', file=sys.stderr)
+        sys.exit(2)
     import numpy as np
     import torch
 
@@ -352,8 +355,6 @@ def main():
     dev = torch.device('cuda', 0)
     torch.cuda.set_device(dev)
     B, S = a.frames, a.streams
-    if B < 1 or S < 1:
-        sys.exit('bench.py: --frames and --streams take positive counts')
     N = S * B  # frames per step
     wa = win2winaux((H, W), CSF, OVERLAP)
     plans = [OlaPlan((H, W), SF, wa, mode='valid', device=dev, max_frames=B) for _ in range(S)]

@@ -61,6 +61,16 @@ def test_bench_refuses_more_than_one_gpu():
     assert r.returncode != 0 and 'one GPU' in r.stderr
 
 
+@pytest.mark.parametrize('args', [['--streams', '0'], ['--frames', '0']])
+def test_bench_refuses_empty_step(args):
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, 'bench.py'] + args, capture_output=True, text=True, cwd=root)
+    assert r.returncode == 2 and 'positive counts' in r.stderr
+
+
 def test_gridf_and_imwrite_roundtrip(tmp_path):
     from vmbd_amd.imagetools import gridF, imread, imwrite
     fs = np.random.default_rng(0).random((9, 5, 5))
