@@ -432,6 +432,24 @@ def main():
         ms = timed(graph.replay, a.steps)
     mpix = N * H * W / (ms * 1e-3) / 1e6
 
+    # one batch alone on one stream, graph-replayed: the step without the
+    # overlap of the batches
+    one_batch = None
+    if graph is not None and S > 1:
+        s1 = torch.cuda.Stream()
+        s1.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s1):
+            batch(0)
+            graph1 = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(graph1, stream=s1):
+                batch(0)
+        torch.cuda.current_stream().wait_stream(s1)
+        for _ in range(a.warmup):
+            graph1.replay()
+        ms_b = timed(graph1.replay, a.steps)
+        one_batch = {'frames': B, 'ms': round(ms_b, 4), 'mpix_s': round(B * H * W / (ms_b * 1e-3) / 1e6, 2)}
+        del graph1
+
     # single-frame eager step, for reference
     plan1 = OlaPlan((H, W), SF, wa, mode='valid', device=dev, max_frames=1)
 
@@ -561,6 +579,7 @@ def main():
         'ms_per_frame': round(ms / N, 4),
         'eager_batched_ms_per_step': round(eager_ms, 4),
         'single_frame_eager': {'ms': round(ms1, 4), 'mpix_s': round(H * W / (ms1 * 1e-3) / 1e6, 2)},
+        'one_batch_one_stream': one_batch,
         'step_alg_bytes': op_bytes,
         'roofline': roofline,
         'kernels': kernels,
