@@ -20,10 +20,10 @@ def bench():
 
 
 def test_operator_bytes_closed_form(bench):
-    # x (1024^2) read by the forward and written by the x-adjoint, y / g
-    # (984^2) written by the forward and read by both adjoints, f (100 41^2
-    # PSFs) read by the forward and written by the f-adjoint: each array
-    # moves 3 times 4 bytes over the three operators
+    # x (1024^2): read by the forward and the f-adjoint, written (x-bar) by
+    # the x-adjoint; y / g (984^2): written by the forward, read by both
+    # adjoints; f (100 PSFs of 41^2): read by the forward and the x-adjoint,
+    # written (f-bar) by the f-adjoint.  Each moves 3 x 4 bytes per value.
     x, y, f = 1024 * 1024, 984 * 984, 100 * 41 * 41
     assert bench.operator_bytes() == 12.0 * (x + y + f) == 26_219_184.0
 
